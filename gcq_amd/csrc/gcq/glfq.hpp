@@ -1,0 +1,266 @@
+// glfq.hpp — G-LFQ: lock-free bounded MPMC ring (SCQ) with one aggregated
+// ticket FAA per wave, plus the two-ring index indirection for u64 payloads.
+//
+// Algorithm follows the reference glf_queue.hpp (SCQ, Nikolaev DISC'19):
+//   entry  = Cycle(8)@48 | IsSafe@47 | Index(32)       glf_queue.hpp:56-90
+//   enqueue / dequeue / catchup                        glf_queue.hpp:447-560
+//   indirection enqueue_ptr / dequeue_ptr              glf_queue.hpp:566-582
+// MI355X layout (own design): the ring header is one 2 KiB block whose
+// 128-byte lines hold the immutable geometry and, on separate lines, Tail,
+// Head and Threshold (the reference packs all three with Entry[0..4] into one
+// line).  Slots follow the header.  Capacity is a runtime field so one binary
+// serves every capacity.
+#pragma once
+#include "prim.hpp"
+
+namespace gcq {
+namespace lfq {
+
+constexpr uint32_t kBot = 0xFFFFFFFEu;   // empty slot
+constexpr uint32_t kBotC = 0xFFFFFFFFu;  // consumed slot
+constexpr int kCycShift = 48;
+constexpr int kSafeShift = 47;
+constexpr uint64_t kSafe = 1ull << kSafeShift;
+constexpr uint64_t kIdxMask = 0xFFFFFFFFull;
+constexpr uint32_t kHdrBytes = 2048;
+
+struct Geometry {
+  uint32_t n;            // capacity, power of two
+  uint32_t slots_log2;   // log2(2n)
+  uint32_t remap_log2;   // log2(slots per 128-B line) when remapping, else 0
+  int32_t thresh_max;    // 3n - 1
+  uint32_t off_tail, off_head, off_thr;  // byte offsets of the counters
+  uint32_t magic;
+};
+constexpr uint32_t kMagic = 0x4C465131u;  // "LFQ1"
+
+// Ring = header + 2n slots.  Device code addresses it through this view.
+struct Ring {
+  Geometry g;
+  uint8_t pad_[kHdrBytes - sizeof(Geometry)];
+};
+static_assert(sizeof(Ring) == kHdrBytes, "ring header size");
+
+__host__ __device__ constexpr uint64_t ring_bytes(uint32_t n) {
+  return (uint64_t)kHdrBytes + 2ull * n * 8ull;
+}
+
+__device__ __forceinline__ uint64_t* slots(Ring* r) { return (uint64_t*)((char*)r + kHdrBytes); }
+__device__ __forceinline__ uint64_t* tail(Ring* r) { return (uint64_t*)((char*)r + r->g.off_tail); }
+__device__ __forceinline__ uint64_t* head(Ring* r) { return (uint64_t*)((char*)r + r->g.off_head); }
+__device__ __forceinline__ int32_t* thr(Ring* r) { return (int32_t*)((char*)r + r->g.off_thr); }
+
+__host__ __device__ __forceinline__ uint32_t e_cyc(uint64_t e) { return (uint32_t)(e >> kCycShift) & 0xFFu; }
+__host__ __device__ __forceinline__ bool e_safe(uint64_t e) { return (e >> kSafeShift) & 1u; }
+__host__ __device__ __forceinline__ uint32_t e_idx(uint64_t e) { return (uint32_t)(e & kIdxMask); }
+__host__ __device__ __forceinline__ uint64_t e_pack(uint32_t cyc, bool safe, uint32_t idx) {
+  return ((uint64_t)(cyc & 0xFFu) << kCycShift) | ((uint64_t)safe << kSafeShift) | (uint64_t)idx;
+}
+constexpr uint64_t kInit = kSafe | (uint64_t)kBot;  // cycle 0, safe, empty
+
+// Three-way modular cycle order on 8 bits (reference glf_queue.hpp:106-110).
+__host__ __device__ __forceinline__ bool cyc_behind(uint32_t a, uint32_t b) {
+  return ((a - b) & 0xFFu) >= 0x80u;
+}
+__host__ __device__ __forceinline__ uint32_t cycle_of(uint64_t t, uint32_t slots_log2) {
+  return (uint32_t)(t >> slots_log2) & 0xFFu;
+}
+// Slot of a ticket.  With remapping, consecutive tickets of one wave land on
+// distinct 128-B lines (reference cache_remap, glf_queue.hpp:116-120, used 64-B
+// lines; MI355X lines hold 16 slots).
+__host__ __device__ __forceinline__ uint32_t slot_of(uint64_t t, uint32_t slots_log2, uint32_t remap_log2) {
+  const uint32_t pos = (uint32_t)t & ((1u << slots_log2) - 1u);
+  if (remap_log2 == 0) return pos;
+  const uint32_t e = 1u << remap_log2;
+  return ((pos & (e - 1u)) << (slots_log2 - remap_log2)) | (pos >> remap_log2);
+}
+
+// ---------------------------------------------------------------- init
+// Grid-stride initialisation: every thread of the launch calls it with its
+// own (gid, stride).  A single thread may call it with (0, 1).
+__device__ inline void init_slice(Ring* r, uint32_t n, bool full, uint32_t remap_log2, uint32_t gid,
+                                  uint32_t stride) {
+  const uint32_t slots_log2 = 31u - __builtin_clz(2u * n);
+  const uint32_t nslots = 2u * n;
+  uint64_t* e = slots(r);
+  for (uint32_t pos = gid; pos < nslots; pos += stride) {
+    const uint64_t t = (uint64_t)nslots + pos;
+    const uint32_t j = slot_of(t, slots_log2, remap_log2);
+    e[j] = (full && pos < n) ? e_pack(cycle_of(t, slots_log2), true, pos) : kInit;
+  }
+  if (gid == 0) {
+    Geometry g;
+    g.n = n;
+    g.slots_log2 = slots_log2;
+    g.remap_log2 = remap_log2;
+    g.thresh_max = 3 * (int32_t)n - 1;
+    g.off_tail = 1 * kLine;
+    g.off_head = 5 * kLine;
+    g.off_thr = 9 * kLine;
+    g.magic = kMagic;
+    r->g = g;
+    *(uint64_t*)((char*)r + g.off_tail) = (uint64_t)nslots + (full ? n : 0u);
+    *(uint64_t*)((char*)r + g.off_head) = (uint64_t)nslots;
+    *(int32_t*)((char*)r + g.off_thr) = full ? g.thresh_max : -1;
+  }
+}
+
+// ---------------------------------------------------------------- ring ops
+// Raise Tail to at least h so enqueuers skip tickets dequeuers already
+// passed.  The reference retries a CAS from a possibly stale Tail up to 32
+// times (glf_queue.hpp:447-455); every failed CAS is another request to the
+// hottest line of the ring.  Tail is a plain counter here, so one fetch_max
+// has the same effect.
+__device__ __forceinline__ void catchup(Ring* r, uint64_t h) { max_rlx(tail(r), h); }
+
+// Head ticket bounded by the items present (consumer side).  The wave
+// leader reads Head, then Tail, and takes k = min(active lanes, Tail - Head)
+// tickets with one FAA: consumers never take a ticket no enqueuer has taken
+// yet (such a ticket marks its slot and sends the enqueuer that later draws it
+// round again).  The reference takes the ticket first (glf_queue.hpp:510-560),
+// so tens of thousands of idle consumers drive Head far past Tail.
+// Returns this lane's ticket, kNoTicket (none this round) or kEmptyTicket
+// (Tail <= Head at the leader's Tail read: EMPTY for the whole wave,
+// linearised at that read).
+constexpr uint64_t kNoTicket = ~0ull, kEmptyTicket = ~1ull;
+__device__ __forceinline__ uint64_t claim_head(Ring* r) {
+  const uint64_t m = active_mask();
+  const uint32_t leader = first_lane(m);
+  const uint32_t n = (uint32_t)__builtin_popcountll(m);
+  uint64_t base = 0;
+  uint32_t k = 0;
+  if (lane_id() == leader) {
+    const uint64_t h = ld_rlx(head(r));
+    vm_wait();  // Head is read before Tail
+    const uint64_t t = ld_rlx(tail(r));
+    const uint64_t avail = t > h ? t - h : 0;
+    k = avail < n ? (uint32_t)avail : n;
+    if (k) base = faa_rlx(head(r), (uint64_t)k);
+  }
+  k = bcast32(k, leader);
+  if (k == 0) return kEmptyTicket;
+  base = bcast64(base, leader);
+  const uint32_t rank = rank_in(m);
+  return rank < k ? base + rank : kNoTicket;
+}
+
+struct NoProbe {
+  __device__ __forceinline__ void step() {}
+  __device__ __forceinline__ void after_ticket() {}
+  __device__ __forceinline__ void slow_entry() {}
+};
+
+// Enqueue a slot index; the ring must have room (indirection guarantees it).
+template <class Probe = NoProbe>
+__device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
+  const Geometry g = r->g;
+  uint64_t* const e = slots(r);
+  for (;;) {
+    const uint64_t T = wave_faa(tail(r), (uint64_t)1);
+    // Every earlier memory operation of this wave completes before the slot
+    // is touched; this is what orders a payload store before the index is
+    // published (and a payload load before the index is recycled).
+    vm_wait();
+    if (pr) { pr->step(); pr->after_ticket(); }
+    const uint32_t tcyc = cycle_of(T, g.slots_log2);
+    const uint32_t j = slot_of(T, g.slots_log2, g.remap_log2);
+    uint64_t E = ld_rlx(&e[j]);
+    for (;;) {
+      if (cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || ld_rlx(head(r)) <= T)) {
+        if (!cas_rlx(&e[j], E, e_pack(tcyc, true, idx))) {  // E refreshed
+          if (pr) pr->step();
+          continue;
+        }
+        if (ld_rlx(thr(r)) != g.thresh_max) xchg_rlx(thr(r), g.thresh_max);
+        return;
+      }
+      break;
+    }
+  }
+}
+
+// Dequeue a slot index, or kBot when the ring is empty.
+// idle_check: consumer side, tickets bounded by the items present; the
+// free-index ring of the indirection is rarely empty and takes plain
+// aggregated tickets.
+template <class Probe = NoProbe>
+__device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check = true) {
+  const Geometry g = r->g;
+  uint64_t* const e = slots(r);
+  if (ld_rlx(thr(r)) < 0) return kBot;
+  for (;;) {
+    uint64_t H;
+    if (idle_check) {
+      H = claim_head(r);
+      if (H == kEmptyTicket) return kBot;
+      if (H == kNoTicket) {
+        if (pr) pr->step();
+        continue;
+      }
+    } else {
+      H = wave_faa(head(r), (uint64_t)1);
+    }
+    if (pr) { pr->step(); pr->after_ticket(); }
+    const uint32_t hcyc = cycle_of(H, g.slots_log2);
+    const uint32_t j = slot_of(H, g.slots_log2, g.remap_log2);
+    uint64_t E = ld_rlx(&e[j]);
+    for (;;) {
+      const uint32_t ecyc = e_cyc(E), eidx = e_idx(E);
+      if (ecyc == hcyc && eidx < kBot) {
+        or_rlx(&e[j], (uint64_t)kBotC);  // consume: index := ⊥c, keep cycle/safe
+        return eidx;
+      }
+      const uint64_t nv = (eidx >= kBot) ? e_pack(hcyc, e_safe(E), kBot) : e_pack(ecyc, false, eidx);
+      if (cyc_behind(ecyc, hcyc) && !cas_rlx(&e[j], E, nv)) {
+        if (pr) pr->step();
+        continue;
+      }
+      break;
+    }
+    const uint64_t T = ld_rlx(tail(r));
+    if (T <= H + 1) {
+      catchup(r, H + 1);
+      faa_rlx(thr(r), -1);
+      return kBot;
+    }
+    if (faa_rlx(thr(r), -1) <= 0) return kBot;
+  }
+}
+
+// ---------------------------------------------------------------- mpmc
+// u64 payload queue: aq holds published indices, fq free indices, data the
+// payloads (reference fq_mpmc_t, glf_queue.hpp:134-138).
+struct Mpmc {
+  Ring* aq;
+  Ring* fq;
+  uint64_t* data;
+};
+__host__ __device__ inline uint64_t mpmc_bytes(uint32_t n) { return 2 * ring_bytes(n) + 8ull * n; }
+__host__ __device__ inline Mpmc mpmc_view(void* base, uint32_t n) {
+  Mpmc m;
+  m.aq = (Ring*)base;
+  m.fq = (Ring*)((char*)base + ring_bytes(n));
+  m.data = (uint64_t*)((char*)base + 2 * ring_bytes(n));
+  return m;
+}
+
+template <class Probe = NoProbe>
+__device__ inline bool enqueue_ptr(const Mpmc& q, uint64_t v, Probe* pr = nullptr) {
+  const uint32_t idx = dequeue(q.fq, pr, false);
+  if (idx == kBot) return false;  // full
+  st_rlx(&q.data[idx], v);  // acknowledged before enqueue() touches a slot
+  enqueue(q.aq, idx, pr);
+  return true;
+}
+
+template <class Probe = NoProbe>
+__device__ inline bool dequeue_ptr(const Mpmc& q, uint64_t* out, Probe* pr = nullptr) {
+  const uint32_t idx = dequeue(q.aq, pr);
+  if (idx == kBot) return false;  // empty
+  *out = ld_rlx(&q.data[idx]);  // address-dependent on the dequeued index;
+  enqueue(q.fq, idx, pr);        // completes before enqueue() touches a slot
+  return true;
+}
+
+}  // namespace lfq
+}  // namespace gcq

@@ -1,0 +1,659 @@
+// gwfq.hpp — G-WFQ: wait-free bounded MPMC ring (wCQ fast path + cooperative
+// slow path with 64-bit CAS only), wave64-native, plus the u64 indirection.
+//
+// Algorithm follows the reference gwf_queue.hpp / gwf_ring.hpp:
+//   global word   Counter<<B | ThrIdx                     gwf_queue.hpp:31-92
+//   fast path     try_enq / try_deq                       gwf_ring.hpp:382-472
+//   consume/finalize/catchup                              gwf_ring.hpp:309-371
+//   slow path     slow_faa, try_{enq,deq}_slow, drivers   gwf_ring.hpp:486-802
+//   helping       help_enqueue/help_dequeue/help_threads  gwf_ring.hpp:808-861
+//   top level     Enqueue_wf / Dequeue_wf                 gwf_ring.hpp:1234-1356
+//   indirection   Enqueue_Ptr / Dequeue_Ptr               gwf_ring.hpp:1362-1380
+//
+// Own MI355X design choices (each documented where it is made):
+//  * slot word = Note(8)@56 | Cycle(8)@48 | Safe@47 | Enq@46 | Owner(20)@26 |
+//    Index(26).  Owner records the requesting thread of a slow-path
+//    (Enq=0) insertion so that consume() finalises it with one CAS instead
+//    of scanning every thread record (reference gwf_ring.hpp:310-328).
+//  * Tail/Head/Threshold on separate 128-B lines of a 2 KiB header.
+//  * help countdown lives in LDS per wave and the round-robin peer cursor in
+//    a per-wave global word; one wave inspects up to 64 peers per help turn
+//    (reference: one peer per thread, countdown stored in global memory and
+//    rewritten on every operation).
+//  * thread-index width B and capacity are runtime geometry (B=17 supports
+//    65,536 threads; the reference's 16-bit preset caps at 65,535).
+//  * try_enq_slow treats a slot carrying our cycle with an EMPTY index as
+//    "a dequeuer passed this ticket" and retries with a fresh ticket; the
+//    reference returns success there (gwf_ring.hpp:697-704), which drops the
+//    item when a fast dequeuer marked the slot first.
+#pragma once
+#include "prim.hpp"
+
+namespace gcq {
+namespace gwf {
+
+// ------------------------------------------------------------------ slot word
+constexpr uint32_t kIdxBits = 26;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+constexpr uint32_t kBot = kIdxMask - 1u;   // ⊥  empty
+constexpr uint32_t kBotC = kIdxMask;       // ⊥c consumed
+constexpr uint32_t kMaxIndex = kBot - 1u;  // largest storable index
+constexpr int kOwnerShift = 26;
+constexpr uint32_t kOwnerMask = (1u << 20) - 1u;
+constexpr int kEnqShift = 46, kSafeShift = 47, kCycShift = 48, kNoteShift = 56;
+constexpr uint64_t kEnqBit = 1ull << kEnqShift;
+constexpr uint64_t kSafeBit = 1ull << kSafeShift;
+constexpr uint64_t kConsume = kEnqBit | (uint64_t)kBotC;  // OR-mask of consume()
+
+__host__ __device__ __forceinline__ uint32_t e_note(uint64_t e) { return (uint32_t)(e >> kNoteShift) & 0xFFu; }
+__host__ __device__ __forceinline__ uint32_t e_cyc(uint64_t e) { return (uint32_t)(e >> kCycShift) & 0xFFu; }
+__host__ __device__ __forceinline__ bool e_safe(uint64_t e) { return (e >> kSafeShift) & 1u; }
+__host__ __device__ __forceinline__ bool e_enq(uint64_t e) { return (e >> kEnqShift) & 1u; }
+__host__ __device__ __forceinline__ uint32_t e_owner(uint64_t e) { return (uint32_t)(e >> kOwnerShift) & kOwnerMask; }
+__host__ __device__ __forceinline__ uint32_t e_idx(uint64_t e) { return (uint32_t)e & kIdxMask; }
+__host__ __device__ __forceinline__ uint64_t e_pack(uint32_t note, uint32_t cyc, bool safe, bool enq,
+                                                    uint32_t owner, uint32_t idx) {
+  return ((uint64_t)(note & 0xFFu) << kNoteShift) | ((uint64_t)(cyc & 0xFFu) << kCycShift) |
+         ((uint64_t)safe << kSafeShift) | ((uint64_t)enq << kEnqShift) |
+         ((uint64_t)(owner & kOwnerMask) << kOwnerShift) | (uint64_t)(idx & kIdxMask);
+}
+constexpr uint64_t kInit = (0xFFull << kNoteShift) | kSafeBit | kEnqBit | (uint64_t)kBot;
+
+__host__ __device__ __forceinline__ bool cyc_behind(uint32_t a, uint32_t b) { return ((a - b) & 0xFFu) >= 0x80u; }
+
+// ------------------------------------------------------------------ local words
+// Per-request local counter: raw<<2 | INC(bit1) | FIN(bit0)   gwf_queue.hpp:202-216
+constexpr uint64_t kFin = 1ull, kInc = 2ull, kFlags = 3ull;
+
+// ------------------------------------------------------------------ records
+// Thread record, one 128-B line (reference thrdrec_t, gwf_queue.hpp:258-288).
+struct alignas(128) Record {
+  uint32_t p2_seq1;   // phase-2 seqlock: bumped before the fields
+  uint32_t p2_seq2;   //                  set equal after the fields
+  uint64_t p2_local;  // address of the local word being incremented
+  uint64_t p2_cnt;    // its counter value (local format)
+  uint32_t seq1;      // help-request seqlock
+  uint32_t seq2;
+  uint32_t enqueue;   // 1 = enqueue request, 0 = dequeue request
+  uint32_t pending;   // request published and not finished
+  uint32_t index;     // index being enqueued
+  uint32_t pad0;
+  uint64_t localTail, initTail, localHead, initHead;
+  uint8_t pad1[128 - 80];
+};
+static_assert(sizeof(Record) == 128, "record must be one line");
+
+constexpr uint32_t kHdrBytes = 2048;
+struct Geometry {
+  uint32_t n, slots_log2, remap_log2, thr_bits;
+  uint32_t max_threads;  // valid thread ids are [0, max_threads)
+  int32_t thresh_max;
+  uint32_t help_delay, enq_patience, deq_patience;
+  uint32_t off_tail, off_head, off_thr;
+  uint64_t off_records, off_cursor;  // records; per-wave peer cursors
+  uint32_t magic, pad;
+};
+constexpr uint32_t kMagic = 0x47574631u;  // "GWF1"
+struct Ring {
+  Geometry g;
+  uint8_t pad_[kHdrBytes - sizeof(Geometry)];
+};
+static_assert(sizeof(Ring) == kHdrBytes, "ring header size");
+
+__host__ __device__ constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+__host__ __device__ constexpr uint64_t records_off(uint32_t n) { return (uint64_t)kHdrBytes + 16ull * n; }
+__host__ __device__ constexpr uint64_t cursor_off(uint32_t n, uint32_t max_threads) {
+  return records_off(n) + 128ull * max_threads;
+}
+__host__ __device__ constexpr uint64_t ring_bytes(uint32_t n, uint32_t max_threads) {
+  return align_up(cursor_off(n, max_threads) + 4ull * ((max_threads + 63) / 64), 128);
+}
+// Thread-index width for a thread ceiling: NULL id = 2^B - 1 must exceed every id.
+__host__ __device__ constexpr uint32_t thr_bits_for(uint32_t max_threads) {
+  uint32_t b = 5;
+  while (((1u << b) - 1u) < max_threads) ++b;
+  return b;
+}
+
+__device__ __forceinline__ uint64_t* slots(Ring* r) { return (uint64_t*)((char*)r + kHdrBytes); }
+__device__ __forceinline__ uint64_t* tail(Ring* r) { return (uint64_t*)((char*)r + r->g.off_tail); }
+__device__ __forceinline__ uint64_t* head(Ring* r) { return (uint64_t*)((char*)r + r->g.off_head); }
+__device__ __forceinline__ int32_t* thr(Ring* r) { return (int32_t*)((char*)r + r->g.off_thr); }
+__device__ __forceinline__ Record* rec(Ring* r, uint32_t tid) {
+  return (Record*)((char*)r + r->g.off_records) + tid;
+}
+__device__ __forceinline__ uint32_t* cursor(Ring* r, uint32_t wave) {
+  return (uint32_t*)((char*)r + r->g.off_cursor) + wave;
+}
+__host__ __device__ __forceinline__ uint32_t cycle_of(uint64_t t, uint32_t slots_log2) {
+  return (uint32_t)(t >> slots_log2) & 0xFFu;
+}
+__host__ __device__ __forceinline__ uint32_t slot_of(uint64_t t, uint32_t slots_log2, uint32_t remap_log2) {
+  const uint32_t pos = (uint32_t)t & ((1u << slots_log2) - 1u);
+  if (remap_log2 == 0) return pos;
+  const uint32_t e = 1u << remap_log2;
+  return ((pos & (e - 1u)) << (slots_log2 - remap_log2)) | (pos >> remap_log2);
+}
+
+// ------------------------------------------------------------------ probes
+// Instrumentation hooks.  NoProbe compiles away; the adversary/step-count
+// kernels pass a probe that counts steps and stalls chosen threads between
+// taking a ticket and touching its slot.
+struct NoProbe {
+  __device__ __forceinline__ void step() {}
+  __device__ __forceinline__ void after_ticket() {}
+  __device__ __forceinline__ void slow_entry() {}
+};
+
+// ------------------------------------------------------------------ init
+struct InitParams {
+  uint32_t n, max_threads, remap_log2, help_delay, enq_patience, deq_patience;
+};
+__device__ inline void init_slice(Ring* r, const InitParams& p, bool full, uint32_t gid, uint32_t stride) {
+  const uint32_t nslots = 2u * p.n;
+  const uint32_t slots_log2 = 31u - __builtin_clz(nslots);
+  const uint32_t B = thr_bits_for(p.max_threads);
+  const uint64_t null_tid = (1ull << B) - 1ull;
+  uint64_t* e = slots(r);
+  for (uint32_t pos = gid; pos < nslots; pos += stride) {
+    const uint64_t t = (uint64_t)nslots + pos;
+    const uint32_t j = slot_of(t, slots_log2, p.remap_log2);
+    e[j] = (full && pos < p.n) ? e_pack(0xFF, cycle_of(t, slots_log2), true, true, 0, pos) : kInit;
+  }
+  Record* rs = (Record*)((char*)r + records_off(p.n));
+  for (uint32_t i = gid; i < p.max_threads; i += stride) {
+    uint64_t* w = (uint64_t*)&rs[i];
+    for (int k = 0; k < 16; ++k) w[k] = 0;
+    rs[i].p2_seq1 = 1;
+    rs[i].seq1 = 1;
+  }
+  uint32_t* cur = (uint32_t*)((char*)r + cursor_off(p.n, p.max_threads));
+  const uint32_t nwaves = (p.max_threads + 63) / 64;
+  for (uint32_t w = gid; w < nwaves; w += stride) cur[w] = (w * 64u + 64u) % p.max_threads;
+  if (gid == 0) {
+    Geometry g;
+    g.n = p.n;
+    g.slots_log2 = slots_log2;
+    g.remap_log2 = p.remap_log2;
+    g.thr_bits = B;
+    g.max_threads = p.max_threads;
+    g.thresh_max = 3 * (int32_t)p.n - 1;
+    g.help_delay = p.help_delay;
+    g.enq_patience = p.enq_patience;
+    g.deq_patience = p.deq_patience;
+    g.off_tail = 1 * kLine;
+    g.off_head = 5 * kLine;
+    g.off_thr = 9 * kLine;
+    g.off_records = records_off(p.n);
+    g.off_cursor = cursor_off(p.n, p.max_threads);
+    g.magic = kMagic;
+    g.pad = 0;
+    r->g = g;
+    const uint64_t t0 = (uint64_t)nslots + (full ? p.n : 0u);
+    *(uint64_t*)((char*)r + g.off_tail) = (t0 << B) | null_tid;
+    *(uint64_t*)((char*)r + g.off_head) = ((uint64_t)nslots << B) | null_tid;
+    *(int32_t*)((char*)r + g.off_thr) = full ? g.thresh_max : -1;
+  }
+}
+
+// ------------------------------------------------------------------ ring ops
+template <class Probe = NoProbe>
+struct RingOps {
+  Ring* r;
+  Geometry g;
+  uint32_t tid;  // caller's thread id, < g.max_threads, unique among users
+  Probe* pr;
+
+  __device__ __forceinline__ RingOps(Ring* ring, uint32_t t, Probe* p) : r(ring), g(ring->g), tid(t), pr(p) {}
+
+  __device__ __forceinline__ uint64_t null_tid() const { return (1ull << g.thr_bits) - 1ull; }
+  __device__ __forceinline__ uint64_t cnt_of(uint64_t w) const { return w >> g.thr_bits; }
+  __device__ __forceinline__ uint32_t tid_of(uint64_t w) const { return (uint32_t)(w & null_tid()); }
+  __device__ __forceinline__ uint64_t pack_g(uint64_t c, uint64_t t) const { return (c << g.thr_bits) | t; }
+  __device__ __forceinline__ uint32_t cyc(uint64_t t) const { return cycle_of(t, g.slots_log2); }
+  __device__ __forceinline__ uint32_t slot(uint64_t t) const { return slot_of(t, g.slots_log2, g.remap_log2); }
+
+  // consume(): finalise a slow-path insertion, then mark the slot consumed
+  // (gwf_ring.hpp:335-341).  The owner field names the requester directly.
+  __device__ __forceinline__ void consume(uint64_t h, uint32_t j, uint64_t E) {
+    if (!e_enq(E)) {
+      const uint64_t hl = h << 2;
+      cas_once(&rec(r, e_owner(E))->localTail, hl, hl | kFin);
+    }
+    or_rlx(&slots(r)[j], kConsume);
+  }
+
+  __device__ __forceinline__ void catchup(uint64_t t, uint64_t h) {  // gwf_ring.hpp:348-371
+    for (int i = 0; i < 32; ++i) {
+      const uint64_t cur = ld_rlx(tail(r));
+      const uint64_t cc = cnt_of(cur);
+      if (cc >= h) break;
+      if (cc != t) {
+        t = cc;
+        h = cnt_of(ld_rlx(head(r)));
+        if (t >= h) break;
+        continue;
+      }
+      if (cas_once(tail(r), cur, pack_g(h, tid_of(cur)))) break;
+      h = cnt_of(ld_rlx(head(r)));
+      t = cnt_of(ld_rlx(tail(r)));
+      if (t >= h) break;
+    }
+  }
+
+  // Fast path enqueue attempt.  0 = done, else the raw ticket that failed.
+  __device__ __forceinline__ uint64_t try_enq(uint32_t idx) {
+    const uint64_t T = cnt_of(wave_faa(tail(r), (uint64_t)1 << g.thr_bits));
+    // Every earlier memory operation of this wave completes before the slot
+    // is touched: this orders a payload store before its index is published
+    // and a payload load before its index is recycled.
+    vm_wait();
+    pr->step();
+    pr->after_ticket();
+    const uint32_t tcyc = cyc(T), j = slot(T);
+    uint64_t* const e = slots(r);
+    uint64_t E = ld_rlx(&e[j]);
+    for (;;) {
+      if (cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot &&
+          (e_safe(E) || cnt_of(ld_rlx(head(r))) <= T)) {
+        if (!cas_rlx(&e[j], E, e_pack(e_note(E), tcyc, true, true, 0, idx))) {
+          pr->step();
+          continue;
+        }
+        if (ld_rlx(thr(r)) != g.thresh_max) xchg_rlx(thr(r), g.thresh_max);
+        return 0;
+      }
+      return T;
+    }
+  }
+
+  // Fast path dequeue attempt.  0 = done (*out = index or kBot), else ticket.
+  // Head ticket bounded by the items present.  The wave leader reads Head,
+  // then Tail, and takes k = min(active lanes, Tail - Head) tickets with one
+  // FAA, so consumers do not take tickets no enqueuer has taken yet (such a
+  // ticket marks its slot and makes the enqueuer that later draws it retry).
+  // Returns this lane's raw ticket, kNoTicket (no ticket this round), or
+  // kEmptyTicket (Tail <= Head at the leader's Tail read: EMPTY, linearised
+  // there, for every lane of the wave).
+  static constexpr uint64_t kNoTicket = ~0ull, kEmptyTicket = ~1ull;
+  __device__ __forceinline__ uint64_t claim_head() {
+    const uint64_t m = active_mask();
+    const uint32_t leader = first_lane(m);
+    const uint32_t n = (uint32_t)__builtin_popcountll(m);
+    uint64_t base = 0;
+    uint32_t k = 0;
+    if (lane_id() == leader) {
+      const uint64_t hw = ld_rlx(head(r));
+      vm_wait();  // Head is read before Tail
+      const uint64_t t = cnt_of(ld_rlx(tail(r))), h = cnt_of(hw);
+      const uint64_t avail = t > h ? t - h : 0;
+      k = avail < n ? (uint32_t)avail : n;
+      if (k) base = cnt_of(faa_rlx(head(r), (uint64_t)k << g.thr_bits));
+    }
+    k = bcast32(k, leader);
+    if (k == 0) return kEmptyTicket;
+    base = bcast64(base, leader);
+    const uint32_t rank = rank_in(m);
+    return rank < k ? base + rank : kNoTicket;
+  }
+
+  // bounded: take the ticket with claim_head() (consumer side); otherwise one
+  // aggregated FAA (the free-index ring of the indirection).
+  __device__ __forceinline__ uint64_t try_deq(uint32_t* out, bool bounded) {
+    uint64_t H;
+    if (bounded) {
+      H = claim_head();
+      if (H == kEmptyTicket) {
+        *out = kBot;
+        return 0;
+      }
+      if (H == kNoTicket) return kNoTicket;
+    } else {
+      H = cnt_of(wave_faa(head(r), (uint64_t)1 << g.thr_bits));
+    }
+    pr->step();
+    pr->after_ticket();
+    const uint32_t hcyc = cyc(H), j = slot(H);
+    uint64_t* const e = slots(r);
+    uint64_t E = ld_rlx(&e[j]);
+    for (;;) {
+      const uint32_t ecyc = e_cyc(E), eidx = e_idx(E);
+      if (ecyc == hcyc && eidx < kBot) {
+        consume(H, j, E);
+        *out = eidx;
+        return 0;
+      }
+      const uint64_t nv = (eidx >= kBot) ? e_pack(e_note(E), hcyc, e_safe(E), true, 0, kBot)
+                                         : (E & ~kSafeBit);  // clear IsSafe, keep the rest
+      if (cyc_behind(ecyc, hcyc) && !cas_rlx(&e[j], E, nv)) {
+        pr->step();
+        continue;
+      }
+      break;
+    }
+    const uint64_t T = cnt_of(ld_rlx(tail(r)));
+    if (T <= H + 1) {
+      catchup(T, H + 1);
+      faa_rlx(thr(r), -1);
+      *out = kBot;
+      return 0;
+    }
+    if (faa_rlx(thr(r), -1) <= 0) {
+      *out = kBot;
+      return 0;
+    }
+    return H;
+  }
+
+  // Publish a phase-2 request in this thread's record (gwf_ring.hpp:486-504).
+  __device__ __forceinline__ void prepare_phase2(uint64_t* local, uint64_t cnt_local) {
+    Record* me = rec(r, tid);
+    const uint32_t seq = ld_rlx(&me->p2_seq1) + 1u;
+    st_rlx(&me->p2_seq1, seq);
+    vm_wait();
+    st_rlx(&me->p2_local, (uint64_t)(uintptr_t)local);
+    st_rlx(&me->p2_cnt, cnt_local);
+    vm_wait();
+    st_rlx(&me->p2_seq2, seq);
+  }
+
+  // Cooperative exactly-once increment of *globalp on behalf of the request
+  // whose local word is *local (gwf_ring.hpp:523-639).  *v is in/out in
+  // local format.  Returns false once the request is finalised (FIN).
+  __device__ bool slow_faa(uint64_t* globalp, uint64_t* local, uint64_t* v, bool is_deq) {
+    uint64_t cnt_local = 0, cnt_raw = 0;
+    bool ok = false;
+    do {
+      pr->step();
+      const uint64_t lc = ld_rlx(local);
+      if (lc & kFin) {
+        *v = lc;
+        return false;
+      }
+      uint64_t gp = ld_rlx(globalp);
+      const uint32_t gt = tid_of(gp);
+      if (gt != (uint32_t)null_tid()) {
+        // Help the pending phase 2 named in the global word: clear INC on
+        // its local word, then clear the thread id from the global word.
+        Record* hr = rec(r, gt);
+        const uint32_t s2 = ld_rlx(&hr->p2_seq2);
+        vm_wait();
+        uint64_t* p2l = (uint64_t*)(uintptr_t)ld_rlx(&hr->p2_local);
+        const uint64_t p2c = ld_rlx(&hr->p2_cnt);
+        vm_wait();
+        const uint32_t s1 = ld_rlx(&hr->p2_seq1);
+        if (s1 == s2 && p2l != nullptr) cas_once(p2l, p2c | kInc, p2c);
+        const uint64_t clean = pack_g(cnt_of(gp), null_tid());
+        if (!cas_once(globalp, gp, clean)) continue;
+        gp = clean;
+      }
+      cnt_raw = cnt_of(gp);
+      cnt_local = cnt_raw << 2;
+      uint64_t vc = *v;
+      if (!cas_rlx(local, vc, cnt_local | kInc)) {
+        *v = vc;
+        if (vc & kFin) return false;
+        if (!(vc & kInc)) return true;  // another helper already advanced it
+        cnt_local = vc & ~kFlags;
+        cnt_raw = cnt_local >> 2;
+      } else {
+        *v = cnt_local | kInc;
+      }
+      prepare_phase2(local, cnt_local);
+      ok = cas_once(globalp, pack_g(cnt_raw, null_tid()), pack_g(cnt_raw + 1, tid));
+    } while (!ok);
+    if (is_deq) faa_rlx(thr(r), -1);
+    cas_once(local, cnt_local | kInc, cnt_local);
+    cas_once(globalp, pack_g(cnt_raw + 1, tid), pack_g(cnt_raw + 1, null_tid()));
+    *v = cnt_local;
+    return true;
+  }
+
+  // One slow-path insertion attempt at ticket T for request `owner`
+  // (gwf_ring.hpp:650-705).
+  __device__ bool try_enq_slow(uint64_t T, uint32_t idx, uint32_t owner) {
+    const uint32_t tcyc = cyc(T), j = slot(T);
+    uint64_t* const e = slots(r);
+    Record* rr = rec(r, owner);
+    uint64_t P = ld_rlx(&e[j]);
+    for (;;) {
+      pr->step();
+      const uint32_t ecyc = e_cyc(P), enote = e_note(P), eidx = e_idx(P);
+      if (cyc_behind(ecyc, tcyc) && cyc_behind(enote, tcyc)) {
+        if ((!e_safe(P) && cnt_of(ld_rlx(head(r))) > T) || eidx < kBot) {
+          // Unusable: advance Note so no other helper produces here.
+          const uint64_t nv = (P & ~(0xFFull << kNoteShift)) | ((uint64_t)tcyc << kNoteShift);
+          if (!cas_rlx(&e[j], P, nv)) continue;
+          return false;
+        }
+        const uint64_t prod = e_pack(enote, tcyc, true, false, owner, idx);  // Enq = 0
+        if (!cas_rlx(&e[j], P, prod)) continue;
+        const uint64_t tl = T << 2;
+        if (cas_once(&rr->localTail, tl, tl | kFin)) cas_once(&e[j], prod, prod | kEnqBit);
+        if (ld_rlx(thr(r)) != g.thresh_max) xchg_rlx(thr(r), g.thresh_max);
+        return true;
+      }
+      if (ecyc != tcyc) return false;
+      return eidx != kBot;  // produced here (maybe consumed); kBot = a dequeuer passed
+    }
+  }
+
+  // One slow-path dequeue attempt at ticket H for request `owner`
+  // (gwf_ring.hpp:712-769).
+  __device__ bool try_deq_slow(uint64_t H, uint32_t owner) {
+    const uint32_t hcyc = cyc(H), j = slot(H);
+    uint64_t* const e = slots(r);
+    Record* rr = rec(r, owner);
+    uint64_t P = ld_rlx(&e[j]);
+    for (;;) {
+      pr->step();
+      const uint32_t ecyc = e_cyc(P), enote = e_note(P), eidx = e_idx(P);
+      if (ecyc == hcyc && eidx < kBot) {
+        const uint64_t hl = H << 2;
+        cas_once(&rr->localHead, hl, hl | kFin);
+        return true;
+      }
+      uint32_t nnote = enote;
+      uint64_t val;
+      if (eidx < kBot) {
+        if (cyc_behind(ecyc, hcyc) && cyc_behind(enote, hcyc)) {
+          const uint64_t nv = (P & ~(0xFFull << kNoteShift)) | ((uint64_t)hcyc << kNoteShift);
+          if (!cas_rlx(&e[j], P, nv)) continue;
+          P = nv;
+          nnote = hcyc;
+        }
+        val = (P & ~kSafeBit);
+        val = (val & ~(0xFFull << kNoteShift)) | ((uint64_t)nnote << kNoteShift);
+      } else {
+        val = e_pack(nnote, hcyc, e_safe(P), true, 0, kBot);
+      }
+      if (cyc_behind(ecyc, hcyc) && !cas_rlx(&e[j], P, val)) continue;
+      const uint64_t T = cnt_of(ld_rlx(tail(r)));
+      if (T <= H + 1) catchup(T, H + 1);
+      if (ld_rlx(thr(r)) < 0) {
+        const uint64_t hl = H << 2;
+        cas_once(&rr->localHead, hl, hl | kFin);
+        return true;
+      }
+      return false;
+    }
+  }
+
+  __device__ void enqueue_slow(uint64_t v, uint32_t idx, uint32_t owner) {
+    Record* rr = rec(r, owner);
+    while (slow_faa(tail(r), &rr->localTail, &v, false)) {
+      if (try_enq_slow(v >> 2, idx, owner)) break;
+    }
+  }
+  __device__ void dequeue_slow(uint64_t v, uint32_t owner) {
+    Record* rr = rec(r, owner);
+    while (slow_faa(head(r), &rr->localHead, &v, true)) {
+      if (try_deq_slow(v >> 2, owner)) break;
+    }
+  }
+
+  // Run the peer's published request if it is still current (seqlock check,
+  // gwf_ring.hpp:808-841).
+  __device__ void help_peer(uint32_t peer) {
+    Record* pr_ = rec(r, peer);
+    if (!ld_rlx(&pr_->pending)) return;
+    const uint32_t s2 = ld_rlx(&pr_->seq2);
+    vm_wait();
+    const uint32_t is_enq = ld_rlx(&pr_->enqueue);
+    const uint32_t idx = ld_rlx(&pr_->index);
+    const uint64_t init = is_enq ? ld_rlx(&pr_->initTail) : ld_rlx(&pr_->initHead);
+    vm_wait();
+    const uint32_t s1 = ld_rlx(&pr_->seq1);
+    if (s1 != s2) return;
+    if (is_enq) enqueue_slow(init, idx, peer);
+    else dequeue_slow(init, peer);
+  }
+
+  // Per-wave helping cadence: every help_delay operations of a wave, each
+  // active lane inspects one peer record, round robin over all threads.
+  __device__ __forceinline__ void help_threads() {
+    __shared__ uint32_t s_countdown[16];  // one per wave of a <=1024-thread block
+    const uint32_t w = (threadIdx.x + threadIdx.y * blockDim.x) / kWave;
+    const uint64_t m = active_mask();
+    const uint32_t leader = first_lane(m);
+    uint32_t c = 0;
+    if (lane_id() == leader) {
+      c = s_countdown[w & 15];
+      if (c == 0 || c > g.help_delay) c = g.help_delay;  // LDS starts undefined
+      c -= 1;
+      s_countdown[w & 15] = c == 0 ? g.help_delay : c;
+    }
+    c = bcast32(c, leader);
+    if (c != 0) return;
+    const uint32_t gw = tid / kWave;
+    uint32_t base = 0;
+    if (lane_id() == leader) {
+      base = ld_rlx(cursor(r, gw));
+      st_rlx(cursor(r, gw), (base + (uint32_t)__builtin_popcountll(m)) % g.max_threads);
+    }
+    base = bcast32(base, leader);
+    const uint32_t peer = (base + rank_in(m)) % g.max_threads;
+    if (peer != tid) help_peer(peer);
+  }
+
+  // Enqueue_wf (gwf_ring.hpp:1234-1284).
+  __device__ void enqueue(uint32_t idx) {
+    help_threads();
+    uint64_t T = 0;
+    for (uint32_t c = g.enq_patience; --c != 0;) {
+      T = try_enq(idx);
+      if (T == 0) return;
+    }
+    pr->slow_entry();
+    Record* me = rec(r, tid);
+    const uint32_t seq = ld_rlx(&me->seq1);
+    const uint64_t tl = T << 2;
+    st_rlx(&me->localTail, tl);
+    st_rlx(&me->initTail, tl);
+    st_rlx(&me->index, idx);
+    st_rlx(&me->enqueue, 1u);
+    vm_wait();
+    st_rlx(&me->seq2, seq);
+    vm_wait();
+    st_rlx(&me->pending, 1u);
+    enqueue_slow(tl, idx, tid);
+    st_rlx(&me->pending, 0u);
+    vm_wait();
+    st_rlx(&me->seq1, seq + 1u);
+  }
+
+  // Dequeue_wf (gwf_ring.hpp:1287-1356).  Returns an index or kBot.
+  // idle_check: consumer-side dequeue with tickets bounded by the items
+  // present (claim_head); the free-index ring of the indirection is rarely
+  // empty and takes plain aggregated tickets.
+  __device__ uint32_t dequeue(bool idle_check = true) {
+    if (ld_rlx(thr(r)) < 0) return kBot;
+    help_threads();
+    uint32_t out = kBot;
+    uint64_t H = 0, last = 0;
+    for (uint32_t c = g.deq_patience; --c != 0;) {
+      H = try_deq(&out, idle_check);
+      if (H == 0) return out;
+      if (H != kNoTicket) last = H;
+    }
+    H = last;  // 0 if no attempt drew a ticket: slow_faa starts from the global counter
+    pr->slow_entry();
+    Record* me = rec(r, tid);
+    const uint32_t seq = ld_rlx(&me->seq1);
+    const uint64_t hl = H << 2;
+    st_rlx(&me->localHead, hl);
+    st_rlx(&me->initHead, hl);
+    st_rlx(&me->enqueue, 0u);
+    vm_wait();
+    st_rlx(&me->seq2, seq);
+    vm_wait();
+    st_rlx(&me->pending, 1u);
+    dequeue_slow(hl, tid);
+    st_rlx(&me->pending, 0u);
+    vm_wait();
+    st_rlx(&me->seq1, seq + 1u);
+    const uint64_t h = ld_rlx(&me->localHead) >> 2;
+    const uint32_t j = slot(h);
+    const uint64_t E = ld_rlx(&slots(r)[j]);
+    if (e_cyc(E) == cyc(h) && e_idx(E) < kBot) {
+      consume(h, j, E);
+      return e_idx(E);
+    }
+    return kBot;
+  }
+};
+
+template <class Probe = NoProbe>
+__device__ __forceinline__ void enqueue(Ring* r, uint32_t tid, uint32_t idx, Probe* p = nullptr) {
+  NoProbe np;
+  RingOps<Probe> ops(r, tid, p ? p : (Probe*)&np);
+  ops.enqueue(idx);
+}
+template <class Probe = NoProbe>
+__device__ __forceinline__ uint32_t dequeue(Ring* r, uint32_t tid, Probe* p = nullptr) {
+  NoProbe np;
+  RingOps<Probe> ops(r, tid, p ? p : (Probe*)&np);
+  return ops.dequeue();
+}
+
+// ------------------------------------------------------------------ mpmc
+struct Mpmc {
+  Ring* aq;
+  Ring* fq;
+  uint64_t* data;
+};
+__host__ __device__ inline uint64_t mpmc_bytes(uint32_t n, uint32_t max_threads) {
+  return 2 * ring_bytes(n, max_threads) + 8ull * n;
+}
+__host__ __device__ inline Mpmc mpmc_view(void* base, uint32_t n, uint32_t max_threads) {
+  Mpmc m;
+  m.aq = (Ring*)base;
+  m.fq = (Ring*)((char*)base + ring_bytes(n, max_threads));
+  m.data = (uint64_t*)((char*)base + 2 * ring_bytes(n, max_threads));
+  return m;
+}
+
+template <class Probe = NoProbe>
+__device__ inline bool enqueue_ptr(const Mpmc& q, uint32_t tid, uint64_t v, Probe* p = nullptr) {
+  NoProbe np;
+  Probe* pp = p ? p : (Probe*)&np;
+  const uint32_t idx = RingOps<Probe>(q.fq, tid, pp).dequeue(false);
+  if (idx == kBot) return false;  // full
+  st_rlx(&q.data[idx], v);  // acknowledged before try_enq touches a slot
+  RingOps<Probe>(q.aq, tid, pp).enqueue(idx);
+  return true;
+}
+
+template <class Probe = NoProbe>
+__device__ inline bool dequeue_ptr(const Mpmc& q, uint32_t tid, uint64_t* out, Probe* p = nullptr) {
+  NoProbe np;
+  Probe* pp = p ? p : (Probe*)&np;
+  const uint32_t idx = RingOps<Probe>(q.aq, tid, pp).dequeue();
+  if (idx == kBot) return false;  // empty
+  *out = ld_rlx(&q.data[idx]);  // address-dependent; completes before try_enq touches a slot
+  RingOps<Probe>(q.fq, tid, pp).enqueue(idx);
+  return true;
+}
+
+}  // namespace gwf
+}  // namespace gcq

@@ -1,0 +1,153 @@
+// prim.hpp — memory-order and wave64 primitives shared by every gcq queue.
+//
+// Every shared access in the queues goes through one of these helpers so that
+// each site states its order explicitly.  All atomics are agent scope (one
+// GPU, all 8 XCDs of an MI355X).  As lowered by hipcc for gfx950:
+//   relaxed RMW      -> global_atomic_* (performed at the coherent L2/EA point)
+//   relaxed load     -> global_load ... sc1          (coherent read)
+//   relaxed store    -> global_store ... sc1         (write-through)
+//   release fence    -> buffer_wbl2 sc1 + s_waitcnt  (publish dirty L2 lines)
+//   acquire fence    -> buffer_inv sc1               (drop stale lines)
+// The reference relies on HIP atomic* being seq_cst (gwf_ring.hpp:15-17);
+// they are relaxed, so publication of non-atomic payload (the mpmc data[]
+// array, ray records) is ordered here with explicit release/acquire.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gcq {
+
+constexpr uint32_t kWave = 64;        // CDNA wavefront width
+constexpr uint32_t kLine = 128;       // MI355X L2 line size in bytes
+
+// ---------------------------------------------------------------- atomics
+// Value parameters are non-deduced so that literal operands convert to the
+// pointee type.
+template <class T>
+struct nd { using type = T; };
+template <class T>
+using nd_t = typename nd<T>::type;
+
+template <class T>
+__device__ __forceinline__ T ld_rlx(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T ld_acq(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_rlx(T* p, nd_t<T> v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_rel(T* p, nd_t<T> v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T faa_rlx(T* p, nd_t<T> v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T or_rlx(T* p, nd_t<T> v) {
+  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T max_rlx(T* p, nd_t<T> v) {
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T xchg_rlx(T* p, nd_t<T> v) {
+  return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// CAS that refreshes `expected` with the observed value on failure.
+template <class T>
+__device__ __forceinline__ bool cas_rlx(T* p, T& expected, nd_t<T> desired) {
+  return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// CAS with release on success: everything stored before it is visible to an
+// agent-scope acquirer that observes the new value.
+template <class T>
+__device__ __forceinline__ bool cas_rel(T* p, T& expected, nd_t<T> desired) {
+  return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELEASE,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ bool cas_once(T* p, nd_t<T> expected, nd_t<T> desired) {
+  return cas_rlx(p, expected, desired);
+}
+
+// Full agent-scope fences.  Measured on MI355X at 65,536 threads: a release
+// or acquire fence costs ~7.4 us per wave (the L2 writeback / invalidate is
+// serialised device-wide) against ~0.08 us for a relaxed store, so they are
+// kept off every hot path (slow-path request publication only).
+__device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_sc() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent"); }
+
+// Completion wait for this lane's outstanding vector-memory operations.
+// Hot-path payload hand-off uses it instead of a fence: payload words are
+// written and read only with agent-scope atomic stores/loads (sc1: performed
+// at the device-coherent level, never left dirty in a per-XCD L2), so once
+// the store is acknowledged it is visible to every XCD, and a consumer's
+// payload load is address-dependent on the index it dequeued.  Waiting for
+// completion therefore orders "payload store -> index publish" and "payload
+// load -> index recycle" without the L2 writeback/invalidate of a fence.
+// The "memory" clobber also stops the compiler moving accesses across it.
+__device__ __forceinline__ void vm_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// ---------------------------------------------------------------- wave64
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t active_mask() { return __ballot(1); }
+__device__ __forceinline__ uint64_t lanes_below() {
+  // __builtin_amdgcn_mbcnt_{lo,hi} counts set bits of a mask below this lane.
+  return (uint64_t)__builtin_amdgcn_mbcnt_hi(0xffffffffu, __builtin_amdgcn_mbcnt_lo(0xffffffffu, 0u));
+}
+__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ uint32_t first_lane(uint64_t mask) { return (uint32_t)__builtin_ctzll(mask); }
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, uint32_t src) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t bcast32(uint32_t v, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+}
+
+// One device-scope fetch-and-add per wave: the active lanes elect the lowest
+// lane, which adds (popcount * step); each lane then receives
+// old + rank * step where rank is its position among the active lanes.
+// Same contract as the reference's aggregated ticket FAA
+// (gwf_ring.hpp:202-223, glf_queue.hpp:392-412).
+template <class T>
+__device__ __forceinline__ T wave_faa(T* p, nd_t<T> step) {
+  const uint64_t m = active_mask();
+  const uint32_t leader = first_lane(m);
+  const uint32_t rank = rank_in(m);
+  T old = 0;
+  if (lane_id() == leader) old = faa_rlx(p, (T)(__builtin_popcountll(m)) * step);
+  old = (T)bcast64((uint64_t)old, leader);
+  return old + (T)rank * step;
+}
+
+// Wave-wide sum of a 64-bit value over active lanes (result valid in all lanes).
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)v, off, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
+// Busy wait of roughly `cycles` shader clocks, used by the injected-delay
+// adversary and by polite polling loops.
+__device__ __forceinline__ void spin_cycles(uint64_t cycles) {
+  const uint64_t t0 = __builtin_readcyclecounter();
+  while (__builtin_readcyclecounter() - t0 < cycles) __builtin_amdgcn_s_sleep(1);
+}
+
+}  // namespace gcq

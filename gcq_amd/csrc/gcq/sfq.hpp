@@ -1,0 +1,206 @@
+// sfq.hpp — SFQ: Scogland & Feng ticket ring (ICPE'15), blocking and
+// non-blocking forms, wave64-native.
+//
+// Algorithm follows the reference sfqueue_hip.{hpp,cpp}:
+//   ticket t -> slot t % QLEN, phase id = 2*(t / QLEN) (+1 for dequeue)
+//   blocking ops  FAA ticket, wait for ids[slot] == id     sfqueue_hip.cpp:55-87
+//   non-blocking  load ticket, check id, CAS ticket        sfqueue_hip.cpp:89-128
+//   closed flag unblocks every wait                        sfqueue_hip.cpp:47-53
+// Own MI355X design choices:
+//  * 64-bit tickets, phase ids and items.  The reference's 32-bit invariant
+//    (sfqueue_hip.hpp:45-47) caps the thread count at 4,095 for QLEN = 2^20;
+//    64-bit phases never wrap, so 65,536 threads run at any capacity.
+//  * slot = {id, item} in one 16-byte pair, so the phase check and the item
+//    transfer touch one line.
+//  * blocking tickets come from one aggregated FAA per wave; non-blocking
+//    claims are wave-batched: the active lanes test consecutive tickets
+//    t0+rank and one CAS per wave claims the ready prefix [t0, t0+k).
+//  * Tail, Head and the closed flag sit on separate 128-B lines.
+#pragma once
+#include "prim.hpp"
+
+namespace gcq {
+namespace sfq {
+
+constexpr int kSuccess = 0, kBusy = 1, kClosed = 2;  // sfqueue_hip.hpp:28-30
+constexpr uint32_t kHdrBytes = 2048;
+
+struct Slot {
+  uint64_t id;
+  uint64_t item;
+};
+
+struct Geometry {
+  uint64_t qlen;
+  uint32_t qlen_log2, nprocs;
+  uint32_t off_tail, off_head, off_closed, magic;
+  uint32_t backoff;  // polite-wait iterations between checks (ref SFQ_BACKOFF_ITERS)
+  uint32_t pad;
+};
+constexpr uint32_t kMagic = 0x53465131u;  // "SFQ1"
+struct Queue {
+  Geometry g;
+  uint8_t pad_[kHdrBytes - sizeof(Geometry)];
+};
+static_assert(sizeof(Queue) == kHdrBytes, "header size");
+
+__host__ __device__ constexpr uint64_t queue_bytes(uint64_t qlen) { return kHdrBytes + 16ull * qlen; }
+
+__device__ __forceinline__ Slot* slots(Queue* q) { return (Slot*)((char*)q + kHdrBytes); }
+__device__ __forceinline__ uint64_t* tail(Queue* q) { return (uint64_t*)((char*)q + q->g.off_tail); }
+__device__ __forceinline__ uint64_t* head(Queue* q) { return (uint64_t*)((char*)q + q->g.off_head); }
+__device__ __forceinline__ uint32_t* closed(Queue* q) { return (uint32_t*)((char*)q + q->g.off_closed); }
+
+struct NoProbe {
+  __device__ __forceinline__ void step() {}
+  __device__ __forceinline__ void after_ticket() {}
+  __device__ __forceinline__ void slow_entry() {}
+};
+
+__device__ inline void init_slice(Queue* q, uint64_t qlen, uint32_t nprocs, uint32_t gid, uint32_t stride) {
+  Slot* s = slots(q);
+  for (uint64_t i = gid; i < qlen; i += stride) {
+    s[i].id = 0;
+    s[i].item = 0;
+  }
+  if (gid == 0) {
+    Geometry g;
+    g.qlen = qlen;
+    g.qlen_log2 = 63u - (uint32_t)__builtin_clzll(qlen);
+    g.nprocs = nprocs;
+    g.off_tail = 1 * kLine;
+    g.off_head = 5 * kLine;
+    g.off_closed = 9 * kLine;
+    g.magic = kMagic;
+    g.backoff = 64;
+    g.pad = 0;
+    q->g = g;
+    *(uint64_t*)((char*)q + g.off_tail) = 0;
+    *(uint64_t*)((char*)q + g.off_head) = 0;
+    *(uint32_t*)((char*)q + g.off_closed) = 0;
+  }
+}
+
+__device__ __forceinline__ bool is_closed(Queue* q) { return ld_rlx(closed(q)) != 0u; }
+__device__ __forceinline__ void close(Queue* q) { st_rlx(closed(q), 1u); }
+
+__device__ __forceinline__ void backoff(uint32_t iters) {
+  for (uint32_t i = 0; i < iters; ++i) __builtin_amdgcn_s_sleep(1);
+}
+
+// Blocking enqueue (sfqueue_hip.cpp:55-70).  `spin_limit` bounds the wait
+// (0 = unbounded); exceeding it returns kBusy so no caller hangs forever.
+template <class Probe = NoProbe>
+__device__ inline int enqueue_blocking(Queue* q, uint64_t item, uint64_t spin_limit = 0, Probe* pr = nullptr) {
+  if (is_closed(q)) return kClosed;
+  const Geometry g = q->g;
+  const uint64_t t = wave_faa(tail(q), (uint64_t)1);
+  if (pr) { pr->step(); pr->after_ticket(); }
+  Slot* s = &slots(q)[t & (g.qlen - 1)];
+  const uint64_t id = 2ull * (t >> g.qlen_log2);
+  uint64_t spins = 0;
+  while (ld_rlx(&s->id) != id) {  // the previous lap's dequeuer has released the slot
+    if (is_closed(q)) return kClosed;
+    if (spin_limit && ++spins > spin_limit) return kBusy;
+    backoff(g.backoff);
+  }
+  st_rlx(&s->item, item);
+  vm_wait();  // item acknowledged before the phase flip publishes it
+  st_rlx(&s->id, id + 1);
+  return kSuccess;
+}
+
+template <class Probe = NoProbe>
+__device__ inline int dequeue_blocking(Queue* q, uint64_t* out, uint64_t spin_limit = 0, Probe* pr = nullptr) {
+  if (is_closed(q)) return kClosed;
+  const Geometry g = q->g;
+  const uint64_t t = wave_faa(head(q), (uint64_t)1);
+  if (pr) { pr->step(); pr->after_ticket(); }
+  Slot* s = &slots(q)[t & (g.qlen - 1)];
+  const uint64_t id = 2ull * (t >> g.qlen_log2) + 1ull;
+  uint64_t spins = 0;
+  while (ld_rlx(&s->id) != id) {
+    if (is_closed(q)) return kClosed;
+    if (spin_limit && ++spins > spin_limit) return kBusy;
+    backoff(g.backoff);
+  }
+  *out = ld_rlx(&s->item);
+  vm_wait();  // item read completes before the slot is handed to the next lap
+  st_rlx(&s->id, id + 1);
+  return kSuccess;
+}
+
+// Wave-batched non-blocking claim on counter *ctr: lanes test consecutive
+// tickets; one CAS claims the ready prefix.  Returns this lane's ticket or
+// ~0 when its op is BUSY.
+__device__ __forceinline__ uint64_t nb_claim(Queue* q, uint64_t* ctr, uint64_t phase_add) {
+  const Geometry g = q->g;
+  const uint64_t m = active_mask();
+  const uint32_t leader = first_lane(m);
+  uint64_t t0 = 0;
+  if (lane_id() == leader) t0 = ld_rlx(ctr);
+  t0 = bcast64(t0, leader);
+  const uint64_t t = t0 + rank_in(m);
+  const uint64_t id = 2ull * (t >> g.qlen_log2) + phase_add;
+  const bool ready = ld_rlx(&slots(q)[t & (g.qlen - 1)].id) == id;
+  const uint64_t not_ready = m & ~__ballot(ready);
+  const uint32_t k = not_ready ? __builtin_popcountll(m & ((1ull << first_lane(not_ready)) - 1ull))
+                               : (uint32_t)__builtin_popcountll(m);
+  uint32_t won = 0;
+  if (lane_id() == leader && k > 0) {
+    uint64_t exp = t0;
+    won = cas_rlx(ctr, exp, t0 + k) ? 1u : 0u;
+  }
+  won = bcast32(won, leader);
+  return (won && rank_in(m) < k) ? t : ~0ull;
+}
+
+// Non-blocking enqueue (sfqueue_hip.cpp:89-107).
+template <class Probe = NoProbe>
+__device__ inline int enqueue_nb(Queue* q, uint64_t item, Probe* pr = nullptr) {
+  if (is_closed(q)) return kClosed;
+  const uint64_t t = nb_claim(q, tail(q), 0);
+  if (pr) pr->step();
+  if (t == ~0ull) return kBusy;
+  Slot* s = &slots(q)[t & (q->g.qlen - 1)];
+  st_rlx(&s->item, item);
+  vm_wait();
+  st_rlx(&s->id, 2ull * (t >> q->g.qlen_log2) + 1ull);
+  return kSuccess;
+}
+
+// Non-blocking dequeue (sfqueue_hip.cpp:109-128).
+template <class Probe = NoProbe>
+__device__ inline int dequeue_nb(Queue* q, uint64_t* out, Probe* pr = nullptr) {
+  if (is_closed(q)) return kClosed;
+  const uint64_t t = nb_claim(q, head(q), 1);
+  if (pr) pr->step();
+  if (t == ~0ull) return kBusy;
+  Slot* s = &slots(q)[t & (q->g.qlen - 1)];
+  *out = ld_rlx(&s->item);
+  vm_wait();
+  st_rlx(&s->id, 2ull * (t >> q->g.qlen_log2) + 2ull);
+  return kSuccess;
+}
+
+// Dequeue that reports EMPTY only when it has observed head >= tail (the
+// queue was empty at the tail read); BUSY attempts are retried.  This is the
+// BFS adapter rule of the reference (bfs_bench.cpp:114-131) and gives
+// dequeue results the FIFO model can check.
+template <class Probe = NoProbe>
+__device__ inline bool dequeue_or_empty(Queue* q, uint64_t* out, uint64_t retry_limit, Probe* pr = nullptr) {
+  for (uint64_t k = 0;; ++k) {
+    const int rc = dequeue_nb(q, out, pr);
+    if (rc == kSuccess) return true;
+    if (rc == kClosed) return false;
+    const uint64_t h = ld_rlx(head(q));
+    vm_wait();  // head is read before tail: head(tail read) >= h >= t means empty
+    const uint64_t t = ld_rlx(tail(q));
+    if (h >= t) return false;
+    if (retry_limit && k >= retry_limit) return false;
+    backoff(8);
+  }
+}
+
+}  // namespace sfq
+}  // namespace gcq

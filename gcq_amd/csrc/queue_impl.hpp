@@ -1,0 +1,110 @@
+// queue_impl.hpp — the gcq_queue handle and the per-kind device adapters
+// that every libgcq kernel (throughput, history, FIFO check, BFS, RT)
+// instantiates.  An adapter is a small by-value struct holding device
+// pointers; enq/deq follow the harness adapter conventions of the reference
+// (throughout_bench.cpp:112-292): enqueue returns false when full, dequeue
+// returns false when empty.
+#pragma once
+#include "common.hpp"
+#include "gcq/glfq.hpp"
+#include "gcq/gwfq.hpp"
+#include "gcq/sfq.hpp"
+#include "gcq/ymc.hpp"
+
+struct gcq_queue {
+  gcq_queue_config cfg;
+  void* dev = nullptr;      // queue object in HBM
+  uint64_t bytes = 0;
+  hipStream_t stream = nullptr;
+  void* aux = nullptr;      // WFQ: per-thread handles
+  uint64_t aux_bytes = 0;
+};
+
+namespace gcqk {
+
+struct GwfQ {
+  gcq::gwf::Mpmc m;
+  template <class P = gcq::gwf::NoProbe>
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr) const {
+    return gcq::gwf::enqueue_ptr<P>(m, tid, v, p);
+  }
+  template <class P = gcq::gwf::NoProbe>
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr) const {
+    return gcq::gwf::dequeue_ptr<P>(m, tid, o, p);
+  }
+};
+
+struct LfqQ {
+  gcq::lfq::Mpmc m;
+  template <class P = gcq::lfq::NoProbe>
+  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr) const {
+    return gcq::lfq::enqueue_ptr<P>(m, v, p);
+  }
+  template <class P = gcq::lfq::NoProbe>
+  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr) const {
+    return gcq::lfq::dequeue_ptr<P>(m, o, p);
+  }
+};
+
+// SFQ: enqueue is one non-blocking attempt (BUSY = legal failed enqueue).
+// Dequeue is one non-blocking attempt in throughput mode (the reference's
+// split harness counts BUSY as empty, throughout_bench.cpp:275-285); with
+// exact_empty set it retries BUSY until it succeeds or observes
+// head >= tail, so an EMPTY result means the queue was empty.
+struct SfqQ {
+  gcq::sfq::Queue* q;
+  uint32_t exact_empty;
+  template <class P = gcq::sfq::NoProbe>
+  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr) const {
+    return gcq::sfq::enqueue_nb(q, v, p) == gcq::sfq::kSuccess;
+  }
+  template <class P = gcq::sfq::NoProbe>
+  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr) const {
+    if (exact_empty) return gcq::sfq::dequeue_or_empty(q, o, 1u << 20, p);
+    return gcq::sfq::dequeue_nb(q, o, p) == gcq::sfq::kSuccess;
+  }
+};
+
+struct YmcQ {
+  gcq::ymc::Queue* q;
+  gcq::ymc::Handle* h;
+  template <class P = gcq::ymc::NoProbe>
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr) const {
+    return gcq::ymc::enqueue(q, h, tid, v, p);
+  }
+  template <class P = gcq::ymc::NoProbe>
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr) const {
+    const uint64_t v = gcq::ymc::dequeue(q, h, tid, p);
+    if (v == gcq::ymc::kEmpty) return false;
+    *o = v;
+    return true;
+  }
+};
+
+}  // namespace gcqk
+
+// Call f(adapter) with the adapter matching the queue kind.
+// exact_empty selects SFQ's checked-empty dequeue (correctness harnesses).
+template <class F>
+int gcq_dispatch(gcq_queue* q, F&& f, bool exact_empty = true) {
+  switch (q->cfg.kind) {
+    case GCQ_GWFQ: {
+      gcqk::GwfQ a{gcq::gwf::mpmc_view(q->dev, q->cfg.capacity, q->cfg.max_threads)};
+      return f(a);
+    }
+    case GCQ_GLFQ: {
+      gcqk::LfqQ a{gcq::lfq::mpmc_view(q->dev, q->cfg.capacity)};
+      return f(a);
+    }
+    case GCQ_SFQ: {
+      gcqk::SfqQ a{(gcq::sfq::Queue*)q->dev, exact_empty ? 1u : 0u};
+      return f(a);
+    }
+    case GCQ_WFQ: {
+      gcqk::YmcQ a{(gcq::ymc::Queue*)q->dev, (gcq::ymc::Handle*)q->aux};
+      return f(a);
+    }
+  }
+  gcqh::set_error("unknown queue kind %d", q->cfg.kind);
+  return GCQ_EINVAL;
+}

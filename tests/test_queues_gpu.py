@@ -1,0 +1,92 @@
+"""GPU parity for the four queues through the C ABI: every emitted history
+must be FIFO-linearizable under the oracle checker (the model of the
+reference's Porcupine checker), the FIFO self-check must verify, and the
+throughput harness must conserve items."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [0, 1, 2, 3]  # G-WFQ, G-LFQ, WFQ, SFQ
+NAMES = {0: "gwfq", 1: "glfq", 2: "wfq", 3: "sfq"}
+
+
+def _queue(gcq, kind, threads, capacity=32768, **kw):
+    return gcq.Queue(kind, capacity=capacity, max_threads=max(threads, 64), **kw)
+
+
+def _check(orc, h, what):
+    rc, msg = orc.lincheck(h, orc.AUTO, 20.0)
+    assert rc == orc.OK, f"{what}: {msg}"
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_config1_history_64x16_alternating(gcq, orc, kind):
+    # 64 threads (block 64), 16 ops/thread alternating enq/deq, capacity 32768.
+    print(f"[hist cfg1] kind={NAMES[kind]}", flush=True)
+    with _queue(gcq, kind, 64) as q:
+        h = q.hist_run(threads=64, ops=16, block=64, mode=0, seed=1)
+    assert len(h) == 64 * 16 * 2
+    enq = h[h["op"] == 0]
+    assert np.all(enq["ret"] == 1) or kind == 3  # SFQ's non-blocking enqueue may report BUSY
+    vals = enq["arg"]
+    assert len(np.unique(vals)) == len(vals)
+    _check(orc, h, f"{NAMES[kind]} cfg1")
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_split_history(gcq, orc, kind):
+    print(f"[hist split] kind={NAMES[kind]}", flush=True)
+    with _queue(gcq, kind, 1024) as q:
+        h = q.hist_run(threads=1024, ops=16, block=256, mode=1, producer_percent=25)
+    _check(orc, h, f"{NAMES[kind]} split")
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_history_65536_threads(gcq, orc, kind):
+    # All four queues at 65,536 threads (the north-star scale); capacity 2^20.
+    print(f"[hist 64k] kind={NAMES[kind]}", flush=True)
+    with gcq.Queue(kind, capacity=1 << 20, max_threads=65536) as q:
+        h = q.hist_run(threads=65536, ops=4, block=256, mode=0)
+    assert len(h) == 65536 * 4 * 2
+    _check(orc, h, f"{NAMES[kind]} 65536 threads")
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_fifo_check_produce_then_drain(gcq, kind):
+    print(f"[fifo] kind={NAMES[kind]}", flush=True)
+    with _queue(gcq, kind, 4096, capacity=1 << 16) as q:
+        r = q.fifo_check(threads=4096, ops=16, block=256, concurrent=False)
+    assert r["verified"], r
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_fifo_check_concurrent_65536(gcq, kind):
+    # 25% producers x 32 ops drained concurrently, capacity 2^20.
+    print(f"[fifo concurrent] kind={NAMES[kind]}", flush=True)
+    with gcq.Queue(kind, capacity=1 << 20, max_threads=65536) as q:
+        r = q.fifo_check(threads=65536, ops=32, block=256, concurrent=True)
+    assert r["in_count"] == 16384 * 32
+    assert r["verified"], r
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_throughput_conserves_items(gcq, kind):
+    print(f"[tb] kind={NAMES[kind]}", flush=True)
+    with _queue(gcq, kind, 4096, capacity=1 << 16) as q:
+        r = q.tb_run(1, threads=4096, chunk_ops=32, producer_percent=25, warmup_ms=20, run_ms=100)
+    assert r["launches"] > 0 and r["elapsed_ms"] > 0
+    assert r["enq_success"] > 0 and r["deq_success"] > 0
+    # items dequeued in the window can exceed those enqueued in it only by
+    # what was queued before the window (at most the capacity)
+    assert r["deq_success"] <= r["enq_success"] + (1 << 16) + 4096 * 32
+
+
+def test_gwfq_slow_path_history(gcq, orc):
+    # Patience 1 sends every operation through the wait-free slow path
+    # (cooperative slow_faa, try_{enq,deq}_slow, helping).
+    print("[hist slow path] gwfq", flush=True)
+    with gcq.Queue(0, capacity=1024, max_threads=256, enq_patience=1, deq_patience=1, help_delay=4) as q:
+        h = q.hist_run(threads=256, ops=16, block=64, mode=0)
+    assert h["slow"].sum() > 0
+    _check(orc, h, "gwfq slow path")
