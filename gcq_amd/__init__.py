@@ -32,13 +32,13 @@ class GcqError(RuntimeError):
 class QueueConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("capacity", C.c_uint32), ("max_threads", C.c_uint32),
                 ("remap", C.c_uint32), ("help_delay", C.c_uint32), ("enq_patience", C.c_uint32),
-                ("deq_patience", C.c_uint32), ("wfq_segment_cells", C.c_uint32)]
+                ("deq_patience", C.c_uint32), ("wfq_segment_cells", C.c_uint32), ("poll_backoff", C.c_uint32)]
 
 
 class FaaConfig(C.Structure):
     _fields_ = [("threads", C.c_uint32), ("block", C.c_uint32), ("counters", C.c_uint32),
                 ("stride_bytes", C.c_uint32), ("aggregate", C.c_uint32), ("iters", C.c_uint32),
-                ("launches", C.c_uint32)]
+                ("launches", C.c_uint32), ("poll_percent", C.c_uint32), ("poll_other_line", C.c_uint32)]
 
 
 class FaaResult(C.Structure):
@@ -226,8 +226,12 @@ def write_jsonl(entries, path):
     _check(load().gcq_hist_write_jsonl(e.ctypes.data, len(e), path.encode()), "gcq_hist_write_jsonl")
 
 
-def faa_peak(threads=65536, block=256, counters=1, stride_bytes=128, aggregate=True, iters=256, launches=10):
-    cfg = FaaConfig(threads, block, counters, stride_bytes, int(aggregate), iters, launches)
+def faa_peak(threads=65536, block=256, counters=1, stride_bytes=128, aggregate=True, iters=256, launches=10,
+             poll_percent=0, poll_other_line=False):
+    """Counter-increment throughput; with poll_percent > 0 that share of the
+    waves only loads (the counter, or another line) as idle pollers would."""
+    cfg = FaaConfig(threads, block, counters, stride_bytes, int(aggregate), iters, launches, poll_percent,
+                    int(poll_other_line))
     res = FaaResult()
     _check(load().gcq_faa_peak(C.byref(cfg), C.byref(res)), "gcq_faa_peak")
     return {f: getattr(res, f) for f, _ in FaaResult._fields_}

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libgcq.so")
 ARCH = os.environ.get("GCQ_OFFLOAD_ARCH", "gfx950")
 
 # Every HIP TU of the library.
-SOURCES = ["core.hip", "bfs.hip", "rt.hip", "serve.hip"]
+SOURCES = ["core.hip", "bfs.hip", "graph.cpp", "rt.hip", "serve.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -22,6 +22,7 @@ HIPCC_FLAGS = [
     "-fPIC",
     "-Wall",
     "-Wno-unused-function",
+    "-pthread",
     # packed-FP32 forms are disabled for all device code (known-bad forms on
     # these boxes); hipcc's host pass warns that it does not know the feature.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -36,13 +37,13 @@ def build(verbose=False):
     srcs = sources()
     objs = []
     for s in srcs:
-        obj = os.path.join(CSRC, "_obj", s.replace(".hip", ".o"))
+        obj = os.path.join(CSRC, "_obj", os.path.splitext(s)[0] + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         cmd = ["hipcc", *HIPCC_FLAGS, "-c", os.path.join(CSRC, s), "-o", obj]
         _run(cmd, verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

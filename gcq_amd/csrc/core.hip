@@ -62,10 +62,10 @@ __global__ void k_init_gwf(void* base, gcq::gwf::InitParams p) {
   for (uint32_t i = gid; i < p.n; i += st) m.data[i] = 0;
 }
 
-__global__ void k_init_lfq(void* base, uint32_t n, uint32_t remap_log2) {
+__global__ void k_init_lfq(void* base, uint32_t n, uint32_t remap_log2, uint32_t poll_backoff) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
   gcq::lfq::Mpmc m = gcq::lfq::mpmc_view(base, n);
-  gcq::lfq::init_slice(m.aq, n, false, remap_log2, gid, st);
+  gcq::lfq::init_slice(m.aq, n, false, remap_log2, gid, st, poll_backoff);
   gcq::lfq::init_slice(m.fq, n, true, remap_log2, gid, st);
   for (uint32_t i = gid; i < n; i += st) m.data[i] = 0;
 }
@@ -103,14 +103,14 @@ int launch_init(gcq_queue* q) {
   switch (c.kind) {
     case GCQ_GWFQ: {
       gcq::gwf::InitParams p{c.capacity, c.max_threads, c.remap ? 4u : 0u, c.help_delay, c.enq_patience,
-                             c.deq_patience};
+                             c.deq_patience, c.poll_backoff};
       hipLaunchKernelGGL(k_init_gwf, dim3(grid_for(std::max(2u * c.capacity, c.max_threads))), dim3(256), 0,
                          q->stream, q->dev, p);
       break;
     }
     case GCQ_GLFQ:
       hipLaunchKernelGGL(k_init_lfq, dim3(grid_for(2u * c.capacity)), dim3(256), 0, q->stream, q->dev,
-                         c.capacity, c.remap ? 4u : 0u);
+                         c.capacity, c.remap ? 4u : 0u, c.poll_backoff);
       break;
     case GCQ_SFQ:
       hipLaunchKernelGGL(k_init_sfq, dim3(grid_for(c.capacity)), dim3(256), 0, q->stream, q->dev,
@@ -157,6 +157,7 @@ extern "C" void gcq_queue_config_default(int kind, gcq_queue_config* c) {
   c->deq_patience = 256;  // WF_DEQ_PATIENCE gwf_queue.hpp:304-306
   if (kind == GCQ_WFQ) c->enq_patience = 16;  // WF_PATIENCE wfqueue_hip_opt.hpp:29-31
   c->wfq_segment_cells = 0;
+  c->poll_backoff = (kind == GCQ_GWFQ || kind == GCQ_GLFQ) ? 2 : 0;
 }
 
 extern "C" int gcq_queue_create(const gcq_queue_config* cfg, gcq_queue** out) {
@@ -249,11 +250,15 @@ extern "C" uint64_t gcq_queue_bytes(const gcq_queue* q) { return q ? q->bytes : 
 namespace {
 
 __global__ void k_faa(uint64_t* base, uint32_t stride_words, uint32_t k, uint32_t iters, uint32_t aggregate,
-                      uint64_t* sink) {
+                      uint32_t poll_pct, uint32_t poll_other, uint64_t* sink) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t* c = base + (uint64_t)((tid / gcq::kWave) % k) * stride_words;
+  const uint32_t wave = tid / gcq::kWave;
+  uint64_t* c = base + (uint64_t)(wave % k) * stride_words;
   uint64_t acc = 0;
-  if (aggregate) {
+  if (wave % 100u < poll_pct) {  // an idle poller: loads only
+    const uint64_t* pl = poll_other ? base + (uint64_t)k * stride_words + 16 : c;
+    for (uint32_t i = 0; i < iters; ++i) acc += gcq::ld_rlx(pl);
+  } else if (aggregate) {
     for (uint32_t i = 0; i < iters; ++i) acc += gcq::wave_faa(c, (uint64_t)1);
   } else {
     for (uint32_t i = 0; i < iters; ++i) acc += gcq::faa_rlx(c, (uint64_t)1);
@@ -280,7 +285,8 @@ extern "C" int gcq_faa_peak(const gcq_faa_config* cfg, gcq_faa_result* res) {
               "threads > 0, block multiple of 64 <= 1024");
   GCQ_REQUIRE(cfg->counters >= 1 && cfg->stride_bytes >= 8 && cfg->stride_bytes % 8 == 0, "bad counters/stride");
   GCQ_REQUIRE(cfg->iters >= 1 && cfg->launches >= 1, "iters/launches >= 1");
-  const uint64_t bytes = (uint64_t)cfg->counters * cfg->stride_bytes + 256;
+  GCQ_REQUIRE(cfg->poll_percent < 100, "poll_percent must be < 100");
+  const uint64_t bytes = (uint64_t)cfg->counters * cfg->stride_bytes + 512;
   uint64_t* d = nullptr;
   GCQ_HIP(hipMalloc(&d, bytes));
   GCQ_HIP(hipMemset(d, 0, bytes));
@@ -289,15 +295,17 @@ extern "C" int gcq_faa_peak(const gcq_faa_config* cfg, gcq_faa_result* res) {
   GCQ_HIP(hipEventCreate(&b));
   const uint32_t grid = (cfg->threads + cfg->block - 1) / cfg->block;
   hipLaunchKernelGGL(k_faa, dim3(grid), dim3(cfg->block), 0, 0, d, cfg->stride_bytes / 8, cfg->counters, cfg->iters,
-                     cfg->aggregate, d + bytes / 8 - 1);  // warm-up launch
+                     cfg->aggregate, cfg->poll_percent, cfg->poll_other_line, d + bytes / 8 - 1);  // warm-up
   GCQ_HIP(hipEventRecord(a, 0));
   for (uint32_t l = 0; l < cfg->launches; ++l)
     hipLaunchKernelGGL(k_faa, dim3(grid), dim3(cfg->block), 0, 0, d, cfg->stride_bytes / 8, cfg->counters,
-                       cfg->iters, cfg->aggregate, d + bytes / 8 - 1);
+                       cfg->iters, cfg->aggregate, cfg->poll_percent, cfg->poll_other_line, d + bytes / 8 - 1);
   GCQ_HIP(hipEventRecord(b, 0));
   GCQ_HIP(hipEventSynchronize(b));
   const double ms = gcqh::ms_between(a, b);
-  const double tickets = (double)grid * cfg->block * cfg->iters * cfg->launches;
+  uint64_t faa_waves = 0;
+  for (uint64_t w = 0; w < (uint64_t)grid * cfg->block / 64; ++w) faa_waves += (w % 100u) >= cfg->poll_percent;
+  const double tickets = (double)faa_waves * 64 * cfg->iters * cfg->launches;
   res->kernel_ms = ms / cfg->launches;
   res->tickets_per_s = tickets / (ms * 1e-3);
   res->atomics_per_s = cfg->aggregate ? res->tickets_per_s / 64.0 : res->tickets_per_s;

@@ -31,6 +31,7 @@ struct Geometry {
   int32_t thresh_max;    // 3n - 1
   uint32_t off_tail, off_head, off_thr;  // byte offsets of the counters
   uint32_t magic;
+  uint32_t poll_backoff;  // consumer-side polite polling base (0 = off)
 };
 constexpr uint32_t kMagic = 0x4C465131u;  // "LFQ1"
 
@@ -79,7 +80,7 @@ __host__ __device__ __forceinline__ uint32_t slot_of(uint64_t t, uint32_t slots_
 // Grid-stride initialisation: every thread of the launch calls it with its
 // own (gid, stride).  A single thread may call it with (0, 1).
 __device__ inline void init_slice(Ring* r, uint32_t n, bool full, uint32_t remap_log2, uint32_t gid,
-                                  uint32_t stride) {
+                                  uint32_t stride, uint32_t poll_backoff = 0) {
   const uint32_t slots_log2 = 31u - __builtin_clz(2u * n);
   const uint32_t nslots = 2u * n;
   uint64_t* e = slots(r);
@@ -98,6 +99,7 @@ __device__ inline void init_slice(Ring* r, uint32_t n, bool full, uint32_t remap
     g.off_head = 5 * kLine;
     g.off_thr = 9 * kLine;
     g.magic = kMagic;
+    g.poll_backoff = poll_backoff;
     r->g = g;
     *(uint64_t*)((char*)r + g.off_tail) = (uint64_t)nslots + (full ? n : 0u);
     *(uint64_t*)((char*)r + g.off_head) = (uint64_t)nslots;
@@ -187,12 +189,16 @@ template <class Probe = NoProbe>
 __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check = true) {
   const Geometry g = r->g;
   uint64_t* const e = slots(r);
-  if (ld_rlx(thr(r)) < 0) return kBot;
+  if (!idle_check && ld_rlx(thr(r)) < 0) return kBot;  // consumer side: the claim tests emptiness
+  if (idle_check) idle_pause(g.poll_backoff);
   for (;;) {
     uint64_t H;
     if (idle_check) {
       H = claim_head(r);
-      if (H == kEmptyTicket) return kBot;
+      if (H == kEmptyTicket) {
+        idle_note(g.poll_backoff, true);
+        return kBot;
+      }
       if (H == kNoTicket) {
         if (pr) pr->step();
         continue;
@@ -204,10 +210,18 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
     const uint32_t hcyc = cycle_of(H, g.slots_log2);
     const uint32_t j = slot_of(H, g.slots_log2, g.remap_log2);
     uint64_t E = ld_rlx(&e[j]);
+    // A bounded ticket lies below Tail: its enqueuer is in flight; give it a
+    // moment before marking the slot.
+    for (uint32_t w = 0; idle_check && w < kSlotWaitSpins && cyc_behind(e_cyc(E), hcyc) && e_idx(E) >= kBot; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      E = ld_rlx(&e[j]);
+      if (pr) pr->step();
+    }
     for (;;) {
       const uint32_t ecyc = e_cyc(E), eidx = e_idx(E);
       if (ecyc == hcyc && eidx < kBot) {
         or_rlx(&e[j], (uint64_t)kBotC);  // consume: index := ⊥c, keep cycle/safe
+        if (idle_check) idle_note(g.poll_backoff, false);
         return eidx;
       }
       const uint64_t nv = (eidx >= kBot) ? e_pack(hcyc, e_safe(E), kBot) : e_pack(ecyc, false, eidx);

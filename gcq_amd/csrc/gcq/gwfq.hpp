@@ -91,7 +91,8 @@ struct Geometry {
   uint32_t help_delay, enq_patience, deq_patience;
   uint32_t off_tail, off_head, off_thr;
   uint64_t off_records, off_cursor;  // records; per-wave peer cursors
-  uint32_t magic, pad;
+  uint32_t magic;
+  uint32_t poll_backoff;  // consumer-side polite polling base (0 = off)
 };
 constexpr uint32_t kMagic = 0x47574631u;  // "GWF1"
 struct Ring {
@@ -147,7 +148,7 @@ struct NoProbe {
 
 // ------------------------------------------------------------------ init
 struct InitParams {
-  uint32_t n, max_threads, remap_log2, help_delay, enq_patience, deq_patience;
+  uint32_t n, max_threads, remap_log2, help_delay, enq_patience, deq_patience, poll_backoff;
 };
 __device__ inline void init_slice(Ring* r, const InitParams& p, bool full, uint32_t gid, uint32_t stride) {
   const uint32_t nslots = 2u * p.n;
@@ -187,7 +188,7 @@ __device__ inline void init_slice(Ring* r, const InitParams& p, bool full, uint3
     g.off_records = records_off(p.n);
     g.off_cursor = cursor_off(p.n, p.max_threads);
     g.magic = kMagic;
-    g.pad = 0;
+    g.poll_backoff = p.poll_backoff;
     r->g = g;
     const uint64_t t0 = (uint64_t)nslots + (full ? p.n : 0u);
     *(uint64_t*)((char*)r + g.off_tail) = (t0 << B) | null_tid;
@@ -316,6 +317,13 @@ struct RingOps {
     const uint32_t hcyc = cyc(H), j = slot(H);
     uint64_t* const e = slots(r);
     uint64_t E = ld_rlx(&e[j]);
+    // A bounded ticket lies below Tail: its enqueuer is in flight; give it a
+    // moment before marking the slot.
+    for (uint32_t w = 0; bounded && w < kSlotWaitSpins && cyc_behind(e_cyc(E), hcyc) && e_idx(E) >= kBot; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      E = ld_rlx(&e[j]);
+      pr->step();
+    }
     for (;;) {
       const uint32_t ecyc = e_cyc(E), eidx = e_idx(E);
       if (ecyc == hcyc && eidx < kBot) {
@@ -567,15 +575,22 @@ struct RingOps {
   // present (claim_head); the free-index ring of the indirection is rarely
   // empty and takes plain aggregated tickets.
   __device__ uint32_t dequeue(bool idle_check = true) {
-    if (ld_rlx(thr(r)) < 0) return kBot;
+    // The consumer side's claim reads Head/Tail itself; the threshold test is
+    // the quick "full" answer for the free-index ring.
+    if (!idle_check && ld_rlx(thr(r)) < 0) return kBot;
+    if (idle_check) idle_pause(g.poll_backoff);
     help_threads();
     uint32_t out = kBot;
     uint64_t H = 0, last = 0;
     for (uint32_t c = g.deq_patience; --c != 0;) {
       H = try_deq(&out, idle_check);
-      if (H == 0) return out;
+      if (H == 0) {
+        if (idle_check) idle_note(g.poll_backoff, out == kBot);
+        return out;
+      }
       if (H != kNoTicket) last = H;
     }
+    if (idle_check) idle_note(g.poll_backoff, false);
     H = last;  // 0 if no attempt drew a ticket: slow_faa starts from the global counter
     pr->slow_entry();
     Record* me = rec(r, tid);

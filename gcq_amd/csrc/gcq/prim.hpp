@@ -9,8 +9,8 @@
 //   release fence    -> buffer_wbl2 sc1 + s_waitcnt  (publish dirty L2 lines)
 //   acquire fence    -> buffer_inv sc1               (drop stale lines)
 // The reference relies on HIP atomic* being seq_cst (gwf_ring.hpp:15-17);
-// they are relaxed, so publication of non-atomic payload (the mpmc data[]
-// array, ray records) is ordered here with explicit release/acquire.
+// they are relaxed, so every cross-thread hand-off here states its order: a
+// completion wait (vm_wait, below) on hot paths, fences on slow paths only.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -133,7 +133,8 @@ __device__ __forceinline__ T wave_faa(T* p, nd_t<T> step) {
   return old + (T)rank * step;
 }
 
-// Wave-wide sum of a 64-bit value over active lanes (result valid in all lanes).
+// Wave-wide sum of a 64-bit value; all 64 lanes must be active (the
+// shuffles read every lane).  The result is valid in every lane.
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
   for (int off = 32; off > 0; off >>= 1) {
     uint32_t lo = __shfl_xor((uint32_t)v, off, 64);
@@ -141,6 +142,38 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
     v += ((uint64_t)hi << 32) | lo;
   }
   return v;
+}
+
+// Bounded wait, used by a dequeuer holding a ticket that an enqueuer has
+// already drawn but not yet written: marking the slot at once would make
+// that enqueuer go round again.  The bound keeps the operation's step count
+// constant-bounded (lock-/wait-freedom are unaffected).
+constexpr uint32_t kSlotWaitSpins = 48;
+
+// Polite polling: idle consumers that keep finding a queue empty keep
+// re-reading its Head/Tail lines, and on MI355X loads of a line that is also
+// the target of device-scope atomics slow those atomics sharply (measured:
+// 25% of the waves polling the counter's line halve the FAA rate; polling
+// another line costs nothing).  Each wave keeps a count of consecutive EMPTY
+// results in LDS and sleeps base * 2^min(count, 6) s_sleep units before its
+// next poll; a success resets it.  LDS holds no value at launch, so the count
+// is masked to 4 bits.
+__device__ __forceinline__ uint8_t* idle_streak() {
+  __shared__ uint8_t s_idle[16];
+  return &s_idle[((threadIdx.x + threadIdx.y * blockDim.x) / kWave) & 15u];
+}
+__device__ __forceinline__ void idle_pause(uint32_t base) {
+  if (!base) return;
+  const uint32_t k = *idle_streak() & 15u;
+  if (!k) return;
+  const uint32_t n = base << (k < 6u ? k : 6u);
+  for (uint32_t i = 0; i < n; ++i) __builtin_amdgcn_s_sleep(1);
+}
+__device__ __forceinline__ void idle_note(uint32_t base, bool empty) {
+  if (!base) return;
+  uint8_t* p = idle_streak();
+  const uint32_t k = *p & 15u;
+  *p = (uint8_t)(empty ? (k < 15u ? k + 1u : 15u) : 0u);
 }
 
 // Busy wait of roughly `cycles` shader clocks, used by the injected-delay

@@ -1,7 +1,37 @@
-"""ctypes bindings for the workload entry points of libgcq (BFS, ray
-tracing, serving).  Registered by gcq_amd.load()."""
+"""ctypes bindings for the workload entry points of libgcq (graphs, BFS,
+batched/served BFS, ray tracing).  Registered by gcq_amd.load()."""
+import ctypes as C
+
+
+class BfsConfig(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("threads", C.c_uint32), ("block", C.c_uint32), ("mode", C.c_int32)]
+
+
+class BfsResult(C.Structure):
+    _fields_ = [("edges_scanned", C.c_uint64), ("reached", C.c_uint64), ("levels", C.c_int32), ("pad", C.c_int32),
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+_P = C.POINTER
+VP = C.c_void_p
+
+SIGNATURES = {
+    "gcq_graph_lattice": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, _P(VP)]),
+    "gcq_graph_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, _P(VP)]),
+    "gcq_graph_from_csr": (C.c_int, [C.c_int32, C.c_int64, VP, VP, _P(VP)]),
+    "gcq_graph_info": (C.c_int, [VP, _P(C.c_int32), _P(C.c_int64)]),
+    "gcq_graph_copy_csr": (C.c_int, [VP, VP, VP]),
+    "gcq_graph_pick_source": (C.c_int32, [VP, C.c_uint64, C.c_uint64]),
+    "gcq_graph_destroy": (C.c_int, [VP]),
+    "gcq_bfs_run": (C.c_int, [VP, C.c_int32, _P(BfsConfig), VP, _P(BfsResult)]),
+}
 
 
 def register(lib):
     """Attach argument/result types for the workload symbols present in lib."""
+    for name, (res, args) in SIGNATURES.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
     return lib

@@ -52,6 +52,7 @@ typedef struct gcq_queue_config {
   uint32_t enq_patience;  /* G-WFQ fast-path attempts + 1 (ref WF_ENQ_PATIENCE 128) */
   uint32_t deq_patience;  /* (ref WF_DEQ_PATIENCE 256) */
   uint32_t wfq_segment_cells; /* WFQ: cells reserved in the segment pool (0 = auto) */
+  uint32_t poll_backoff;  /* G-WFQ/G-LFQ consumer polite-polling base, s_sleep units (0 = off) */
 } gcq_queue_config;
 
 /* Fill a config with the reference defaults for `kind`. */
@@ -82,6 +83,8 @@ typedef struct gcq_faa_config {
   uint32_t aggregate;     /* 1 = one FAA per wave (ballot + prefix), 0 = per lane */
   uint32_t iters;         /* FAAs per thread per launch */
   uint32_t launches;
+  uint32_t poll_percent;  /* waves with (wave % 100) < poll_percent only load, as idle pollers */
+  uint32_t poll_other_line; /* 1 = pollers load a different line than the counters */
 } gcq_faa_config;
 typedef struct gcq_faa_result {
   double tickets_per_s;   /* counter increments per second (all lanes) */
@@ -157,6 +160,49 @@ typedef struct gcq_hist_config {
 int gcq_hist_run(gcq_queue* q, const gcq_hist_config* cfg, gcq_hist_entry* out, size_t cap, size_t* n_out);
 /* Write entries as the reference JSONL schema (history.cpp:271-279). */
 int gcq_hist_write_jsonl(const gcq_hist_entry* e, size_t n, const char* path);
+
+/* ------------------------------------------------------------------ graphs */
+/* CSR int32 graph resident in HBM (a host copy is kept for inspection). */
+typedef struct gcq_graph gcq_graph;
+/* R x C lattice, each lattice edge kept with probability `keep`, plus a short
+ * diagonal per vertex with probability `diag`; the reference builds the full
+ * lattice only (build_grid, bfs_bench.cpp:245-256). */
+int gcq_graph_lattice(int32_t rows, int32_t cols, double keep, double diag, uint64_t seed, gcq_graph** out);
+/* R-MAT, 2^scale vertices, edge_factor * 2^scale generated edges,
+ * symmetrised and deduplicated.  No reference counterpart. */
+int gcq_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                   gcq_graph** out);
+/* Upload a caller CSR (sorted, symmetric) to HBM; replaces the CSR H2D copy
+ * of bfs_bench.cpp:703-708. */
+int gcq_graph_from_csr(int32_t nv, int64_t ne, const int32_t* offsets, const int32_t* indices, gcq_graph** out);
+int gcq_graph_info(const gcq_graph* g, int32_t* nv, int64_t* ne);
+int gcq_graph_copy_csr(const gcq_graph* g, int32_t* offsets, int32_t* indices);
+int32_t gcq_graph_pick_source(const gcq_graph* g, uint64_t seed, uint64_t k);
+int gcq_graph_destroy(gcq_graph* g);
+
+/* ------------------------------------------------------------------ BFS */
+enum gcq_bfs_mode {
+  GCQ_BFS_HOST_LOOP = 0,  /* one launch + host sync per level (bfs_bench.cpp:583-623) */
+  GCQ_BFS_PERSISTENT = 1  /* one launch, device-side level barrier */
+};
+typedef struct gcq_bfs_config {
+  int32_t kind;       /* frontier queue: enum gcq_kind */
+  uint32_t threads;   /* worker threads, multiple of 64 (ref --threads 8192) */
+  uint32_t block;     /* (ref --block 256) */
+  int32_t mode;       /* enum gcq_bfs_mode */
+} gcq_bfs_config;
+typedef struct gcq_bfs_result {
+  uint64_t edges_scanned;  /* sum of degrees of dequeued vertices (ref edges_total) */
+  uint64_t reached;
+  int32_t levels;
+  int32_t pad;
+  double kernel_ms;        /* device time of the traversal (ref kernel_ms) */
+  double total_ms;         /* including queue/level reset (ref total_ms) */
+} gcq_bfs_result;
+/* One BFS from `src` (out of range -> 0, bfs_bench.cpp:555).  depth_out, if
+ * not NULL, receives nv int32 depths (-1 = unreached).  Replaces run_bfs +
+ * bfs_expand_kernel + bfs_hub_kernel (bfs_bench.cpp:327-633). */
+int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg, int32_t* depth_out, gcq_bfs_result* res);
 
 #ifdef __cplusplus
 }

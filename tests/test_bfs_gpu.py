@@ -1,0 +1,108 @@
+"""GPU parity of the queue-driven BFS (bfs_bench.cpp:327-633 semantics)
+through the C ABI: every depth array must equal the serial BFS oracle's
+bit for bit, and reached / edges scanned / levels must match it too."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [0, 1, 2, 3]
+NAMES = {0: "gwfq", 1: "glfq", 2: "wfq", 3: "sfq"}
+MODES = {0: "hostloop", 1: "persistent"}
+
+
+def _csr(nv, u, v):
+    u = np.asarray(u, np.int64)
+    v = np.asarray(v, np.int64)
+    key = np.unique(np.concatenate([u * nv + v, v * nv + u]))
+    s, t = np.divmod(key, nv)
+    off = np.zeros(nv + 1, np.int64)
+    np.add.at(off, s + 1, 1)
+    return np.cumsum(off).astype(np.int32), t.astype(np.int32)
+
+
+def _run_and_check(orc, g, src, kind, threads, mode, off=None, idx=None):
+    print(f"[bfs] kind={NAMES[kind]} mode={MODES[mode]} threads={threads} src={src} nv={g.info()[0]}", flush=True)
+    if off is None:
+        off, idx = g.csr()
+    d, res = g.bfs(src, kind=kind, threads=threads, block=256 if threads >= 256 else 64, mode=mode)
+    ref, reached, es = orc.bfs_serial(off, idx, src)
+    bad = np.flatnonzero(d != ref)
+    assert len(bad) == 0, f"{len(bad)} depths differ, first at {bad[:5]}: got {d[bad[:5]]} want {ref[bad[:5]]}"
+    assert res["reached"] == reached
+    assert res["edges_scanned"] == es
+    assert res["levels"] == int(ref.max()) + 1
+    return res
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=[MODES[m] for m in (0, 1)])
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_lattice_small(gcq, orc, kind, mode):
+    from gcq_amd import bfs
+    with bfs.Graph.lattice(64, 64, 0.75, 0.01, 7) as g:
+        off, idx = g.csr()
+        for threads in (64, 1024):
+            _run_and_check(orc, g, 0, kind, threads, mode, off, idx)
+        _run_and_check(orc, g, g.pick_source(5, 0), kind, 1024, mode, off, idx)
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_special_graphs(gcq, orc, kind):
+    from gcq_amd import bfs
+    n = 3000
+    a = np.arange(n - 1)
+    cases = {
+        "path": (_csr(n, a, a + 1), 0),
+        "path_mid": (_csr(n, a, a + 1), n // 2),
+        "star": (_csr(n, np.zeros(n - 1, int), np.arange(1, n)), 7),  # hub of degree 2999 reached at depth 1
+        "two_parts": (_csr(8, [0, 1, 4, 5], [1, 2, 5, 6]), 4),
+        "isolated_src": (_csr(5, [0, 1], [1, 2]), 4),
+        "single": ((np.array([0, 0], np.int32), np.array([], np.int32)), 0),
+        "src_out_of_range": (_csr(6, [0, 1, 2], [1, 2, 3]), 1000),  # falls back to 0 (bfs_bench.cpp:555)
+    }
+    for name, ((off, idx), src) in cases.items():
+        print(f"[bfs special] {name}", flush=True)
+        with bfs.Graph.from_csr(off, idx) as g:
+            for mode in (0, 1):
+                _run_and_check(orc, g, src, kind, 1024, mode, off, idx)
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_rmat_hubs(gcq, orc, kind):
+    # Scale 15 R-MAT: frontiers well above the reference's 32,768-slot
+    # rings and hub vertices of degree in the thousands.
+    from gcq_amd import bfs
+    with bfs.Graph.rmat(15, 16, 0.57, 0.19, 0.19, 11) as g:
+        off, idx = g.csr()
+        for k in range(2):
+            _run_and_check(orc, g, g.pick_source(11, k), kind, 8192, 1, off, idx)
+        _run_and_check(orc, g, g.pick_source(11, 2), kind, 8192, 0, off, idx)
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_lattice_1000_persistent_reuse(gcq, orc, kind):
+    # One graph, repeated runs reuse the frontier queues (reset in place).
+    from gcq_amd import bfs
+    with bfs.Graph.lattice(1000, 1000, 0.75, 0.01, 7) as g:
+        off, idx = g.csr()
+        for src in (0, 999_999, 500_500):
+            _run_and_check(orc, g, src, kind, 8192, 1, off, idx)
+
+
+def test_config3_lattice_4900(gcq, orc):
+    # The benchmark graph: 4900 x 4900 lattice, p = 0.75, 1% diagonals,
+    # seed 7, source 0, 8,192 threads (block 256), G-LFQ, persistent.
+    from gcq_amd import bfs
+    with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as g:
+        res = _run_and_check(orc, g, 0, 1, 8192, 1)
+        print(f"[bfs cfg3] levels={res['levels']} reached={res['reached']} kernel_ms={res['kernel_ms']:.1f} "
+              f"MTEPS={res['edges_scanned'] / res['kernel_ms'] / 1e3:.1f}", flush=True)
+
+
+def test_kind_switch_same_graph(gcq, orc):
+    from gcq_amd import bfs
+    with bfs.Graph.rmat(12, 16, 0.57, 0.19, 0.19, 11) as g:
+        off, idx = g.csr()
+        src = g.pick_source(3, 0)
+        for kind in (0, 3, 1, 2, 0):
+            _run_and_check(orc, g, src, kind, 2048, 1, off, idx)
