@@ -37,10 +37,9 @@ struct BfsCtx {
   int kind = -1;
   uint32_t threads = 0, capacity = 0;
   uint64_t ymc_cells = 0;
-  void* qa = nullptr;
-  void* qb = nullptr;
-  void* ha = nullptr;  // WFQ handles
-  void* hb = nullptr;
+  int nq = 0;                          // frontier queues (2, or 3 for WFQ)
+  void* q[3] = {nullptr, nullptr, nullptr};
+  void* h[3] = {nullptr, nullptr, nullptr};  // WFQ handles
   int32_t* level = nullptr;
   void* ctl = nullptr;
   uint32_t* pinned = nullptr;  // host-visible counters for the host-loop mode
@@ -71,6 +70,9 @@ constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
 
 // ---------------------------------------------------------------- frontier adapters
 struct FGwf {
+  static constexpr int kQueues = 2;
+  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ void rewind() const {}
   gcq::gwf::Ring* r;
   __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
     gcq::gwf::NoProbe np;
@@ -88,6 +90,9 @@ struct FGwf {
   }
 };
 struct FLfq {
+  static constexpr int kQueues = 2;
+  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ void rewind() const {}
   gcq::lfq::Ring* r;
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     gcq::lfq::enqueue(r, v);
@@ -101,6 +106,9 @@ struct FLfq {
   }
 };
 struct FSfq {
+  static constexpr int kQueues = 2;
+  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ void rewind() const {}
   gcq::sfq::Queue* q;
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     return gcq::sfq::enqueue_blocking(q, (uint64_t)v, 1ull << 22) == gcq::sfq::kSuccess;
@@ -112,9 +120,19 @@ struct FSfq {
     return true;
   }
 };
+// WFQ cells are never reused by the algorithm, and every level burns up to one
+// cell per idle lane, so BFS rotates three WFQ frontiers: while one is the
+// level's input and one its output, the third (drained a level earlier)
+// is cleared by the grid and rewound at the level's barrier.
 struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt.hpp:42-46)
+  static constexpr int kQueues = 3;
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
+  __device__ __forceinline__ void recycle(uint64_t gid, uint64_t stride, uint32_t tid) const {
+    gcq::ymc::recycle_cells(q, gid, stride);
+    gcq::ymc::reset_handle(q, h, tid);
+  }
+  __device__ __forceinline__ void rewind() const { gcq::ymc::rewind(q); }
   __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
     return gcq::ymc::enqueue(q, h, tid, (uint64_t)v + 1);
   }
@@ -205,9 +223,11 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
   if (sc) flush(qout, tid, stage, sc, sc, err);
 }
 
-// Sense-reversing grid barrier over co-resident blocks.  Returns false once
-// any block has aborted (spin limit), so every block leaves its loop.
-__device__ bool grid_barrier(Ctl* c, uint32_t nblocks) {
+// Sense-reversing grid barrier over co-resident blocks.  The last block to
+// arrive runs `last()` before releasing the others.  Returns false once any
+// block has aborted (spin limit), so every block leaves its loop.
+template <class Last>
+__device__ bool grid_barrier(Ctl* c, uint32_t nblocks, Last last) {
   gcq::vm_wait();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -215,6 +235,7 @@ __device__ bool grid_barrier(Ctl* c, uint32_t nblocks) {
     gcq::vm_wait();  // generation read before arriving
     const uint32_t a = gcq::faa_rlx(&c->bar_count, 1u);
     if (a == nblocks - 1) {
+      last();
       gcq::st_rlx(&c->bar_count, 0u);
       gcq::vm_wait();
       gcq::st_rlx(&c->bar_gen, g + 1u);
@@ -244,24 +265,37 @@ __device__ void finish_counts(LevelAcc& acc, uint64_t disc_total, Ctl* c) {
   }
 }
 
+// Frontier queues of one traversal: level L reads q[L % K] and writes
+// q[(L + 1) % K]; with K = 3 the third one is recycled during the level.
 template <class F>
-__global__ void k_persistent(F qa, F qb, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
+struct Frontiers {
+  F q[3];
+  __device__ __forceinline__ const F& in(int32_t L) const { return q[L % F::kQueues]; }
+  __device__ __forceinline__ const F& out(int32_t L) const { return q[(L + 1) % F::kQueues]; }
+  __device__ __forceinline__ const F& idle(int32_t L) const { return q[(L + 2) % 3]; }
+};
+
+template <class F>
+__global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
                              int32_t* level, Ctl* c) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
   uint32_t* st = stage[(threadIdx.x / 64) & 15];
   LevelAcc acc{0, 0};
   uint64_t disc_total = 0;
   int32_t L = 0;
   for (;; ++L) {
-    const F& qin = (L & 1) ? qb : qa;
-    const F& qout = (L & 1) ? qa : qb;
+    if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
     acc.disc = 0;
-    expand_level(qin, qout, offs, idx, level, L, tid, st, acc, &c->err);
+    expand_level(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err);
     disc_total += acc.disc;
     const uint32_t d = wave_sum32(acc.disc);
     if (lane_id() == 0 && d) gcq::faa_rlx(&c->next[L % 3], d);
-    if (!grid_barrier(c, gridDim.x)) break;
+    if (!grid_barrier(c, gridDim.x, [&]() {
+          if (F::kQueues == 3) fq.idle(L).rewind();
+        }))
+      break;
     const uint32_t n = gcq::ld_rlx(&c->next[L % 3]);
     if (blockIdx.x == 0 && threadIdx.x == 0) gcq::st_rlx(&c->next[(L + 2) % 3], 0u);
     if (n == 0) break;
@@ -271,16 +305,23 @@ __global__ void k_persistent(F qa, F qb, const int32_t* __restrict__ offs, const
 }
 
 template <class F>
-__global__ void k_level(F qin, F qout, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
+__global__ void k_level(Frontiers<F> fq, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
                         int32_t* level, int32_t L, Ctl* c) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid == 0) gcq::st_rlx(&c->next[(L + 1) & 1], 0u);  // the next level's counter
+  if (F::kQueues == 3) fq.idle(L).recycle(tid, (uint64_t)gridDim.x * blockDim.x, tid);
   LevelAcc acc{0, 0};
-  expand_level(qin, qout, offs, idx, level, L, tid, stage[(threadIdx.x / 64) & 15], acc, &c->err);
+  expand_level(fq.in(L), fq.out(L), offs, idx, level, L, tid, stage[(threadIdx.x / 64) & 15], acc, &c->err);
   const uint32_t d = wave_sum32(acc.disc);
   if (lane_id() == 0 && d) gcq::faa_rlx(&c->next[L & 1], d);
   finish_counts<F>(acc, acc.disc, c);
+}
+
+// Host-loop mode: rewind the recycled frontier after its level's launch.
+template <class F>
+__global__ void k_rewind(Frontiers<F> fq, int32_t L) {
+  if (threadIdx.x == 0) fq.idle(L).rewind();
 }
 
 // ---------------------------------------------------------------- resets
@@ -307,10 +348,9 @@ __global__ void k_reset_sfq(void* a, void* b, uint64_t n) {
   gcq::sfq::init_slice((gcq::sfq::Queue*)a, n, 0, gid, st);
   gcq::sfq::init_slice((gcq::sfq::Queue*)b, n, 0, gid, st);
 }
-__global__ void k_reset_ymc(void* a, void* ha, void* b, void* hb, uint64_t cells, uint32_t nthreads) {
+__global__ void k_reset_ymc(void* q, void* h, uint64_t cells, uint32_t nthreads) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
-  gcq::ymc::init_slice((gcq::ymc::Queue*)a, (gcq::ymc::Handle*)ha, cells, nthreads, 16, gid, st);
-  gcq::ymc::init_slice((gcq::ymc::Queue*)b, (gcq::ymc::Handle*)hb, cells, nthreads, 16, gid, st);
+  gcq::ymc::init_slice((gcq::ymc::Queue*)q, (gcq::ymc::Handle*)h, cells, nthreads, 16, gid, st);
 }
 // Seed the level-0 frontier with the source.
 template <class F>
@@ -334,7 +374,7 @@ uint32_t pow2_at_least(uint64_t x) {
 
 int ctx_free(gcq_graph* g) {
   BfsCtx& c = g->ctx;
-  for (void* p : {c.qa, c.qb, c.ha, c.hb, (void*)c.level, c.ctl})
+  for (void* p : {c.q[0], c.q[1], c.q[2], c.h[0], c.h[1], c.h[2], (void*)c.level, c.ctl})
     if (p) (void)hipFree(p);
   if (c.pinned) (void)hipHostFree(c.pinned);
   c = BfsCtx();
@@ -356,16 +396,17 @@ int ctx_ensure(gcq_graph* g, int kind, uint32_t threads) {
     case GCQ_GLFQ: qbytes = gcq::lfq::ring_bytes(c.capacity); break;
     case GCQ_SFQ: qbytes = gcq::sfq::queue_bytes(c.capacity); break;
     case GCQ_WFQ:
-      c.ymc_cells = 2ull * nv + 64ull * threads + (1ull << 24);
+      // One level's cells: its frontier (<= V) plus at most one burned cell
+      // per dequeuing lane; the queue is recycled two levels later.
+      c.ymc_cells = (uint64_t)nv + 4ull * threads + (1ull << 20);
       qbytes = gcq::ymc::queue_bytes(c.ymc_cells);
       break;
     default: GCQ_REQUIRE(false, "unknown queue kind %d", kind);
   }
-  GCQ_HIP(hipMalloc(&c.qa, qbytes));
-  GCQ_HIP(hipMalloc(&c.qb, qbytes));
-  if (kind == GCQ_WFQ) {
-    GCQ_HIP(hipMalloc(&c.ha, sizeof(gcq::ymc::Handle) * (size_t)threads));
-    GCQ_HIP(hipMalloc(&c.hb, sizeof(gcq::ymc::Handle) * (size_t)threads));
+  c.nq = kind == GCQ_WFQ ? 3 : 2;
+  for (int i = 0; i < c.nq; ++i) {
+    GCQ_HIP(hipMalloc(&c.q[i], qbytes));
+    if (kind == GCQ_WFQ) GCQ_HIP(hipMalloc(&c.h[i], sizeof(gcq::ymc::Handle) * (size_t)threads));
   }
   GCQ_HIP(hipMalloc(&c.level, 4ull * nv));
   GCQ_HIP(hipMalloc(&c.ctl, sizeof(bfsk::Ctl)));
@@ -386,35 +427,34 @@ int reset_run(gcq_graph* g, int32_t src) {
   switch (c.kind) {
     case GCQ_GWFQ: {
       gcq::gwf::InitParams p{c.capacity, c.threads, 0, 64, 128, 256, 0};
-      hipLaunchKernelGGL(bfsk::k_reset_gwf, dim3(grid_of(2ull * c.capacity)), dim3(256), 0, g->stream, c.qa, c.qb,
+      hipLaunchKernelGGL(bfsk::k_reset_gwf, dim3(grid_of(2ull * c.capacity)), dim3(256), 0, g->stream, c.q[0], c.q[1],
                          p);
       hipLaunchKernelGGL(bfsk::k_seed<bfsk::FGwf>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FGwf{(gcq::gwf::Ring*)c.qa}, c.level, src, (bfsk::Ctl*)c.ctl);
+                         bfsk::FGwf{(gcq::gwf::Ring*)c.q[0]}, c.level, src, (bfsk::Ctl*)c.ctl);
       break;
     }
     case GCQ_GLFQ:
-      hipLaunchKernelGGL(bfsk::k_reset_lfq, dim3(grid_of(2ull * c.capacity)), dim3(256), 0, g->stream, c.qa, c.qb,
+      hipLaunchKernelGGL(bfsk::k_reset_lfq, dim3(grid_of(2ull * c.capacity)), dim3(256), 0, g->stream, c.q[0], c.q[1],
                          c.capacity);
       hipLaunchKernelGGL(bfsk::k_seed<bfsk::FLfq>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FLfq{(gcq::lfq::Ring*)c.qa}, c.level, src, (bfsk::Ctl*)c.ctl);
+                         bfsk::FLfq{(gcq::lfq::Ring*)c.q[0]}, c.level, src, (bfsk::Ctl*)c.ctl);
       break;
     case GCQ_SFQ:
-      hipLaunchKernelGGL(bfsk::k_reset_sfq, dim3(grid_of(c.capacity)), dim3(256), 0, g->stream, c.qa, c.qb,
+      hipLaunchKernelGGL(bfsk::k_reset_sfq, dim3(grid_of(c.capacity)), dim3(256), 0, g->stream, c.q[0], c.q[1],
                          (uint64_t)c.capacity);
       hipLaunchKernelGGL(bfsk::k_seed<bfsk::FSfq>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FSfq{(gcq::sfq::Queue*)c.qa}, c.level, src, (bfsk::Ctl*)c.ctl);
+                         bfsk::FSfq{(gcq::sfq::Queue*)c.q[0]}, c.level, src, (bfsk::Ctl*)c.ctl);
       break;
     case GCQ_WFQ: {
-      // zero both cell pools (they are touched up to max(T, H) of the last run)
       const uint64_t words = c.ymc_cells * 2;
-      hipLaunchKernelGGL(bfsk::k_zero_u64, dim3(grid_of(words)), dim3(256), 0, g->stream,
-                         (uint64_t*)((char*)c.qa + gcq::ymc::kHdrBytes), words);
-      hipLaunchKernelGGL(bfsk::k_zero_u64, dim3(grid_of(words)), dim3(256), 0, g->stream,
-                         (uint64_t*)((char*)c.qb + gcq::ymc::kHdrBytes), words);
-      hipLaunchKernelGGL(bfsk::k_reset_ymc, dim3(grid_of(c.threads)), dim3(256), 0, g->stream, c.qa, c.ha, c.qb, c.hb,
-                         c.ymc_cells, c.threads);
+      for (int i = 0; i < 3; ++i) {
+        hipLaunchKernelGGL(bfsk::k_zero_u64, dim3(grid_of(words)), dim3(256), 0, g->stream,
+                           (uint64_t*)((char*)c.q[i] + gcq::ymc::kHdrBytes), words);
+        hipLaunchKernelGGL(bfsk::k_reset_ymc, dim3(grid_of(c.threads)), dim3(256), 0, g->stream, c.q[i], c.h[i],
+                           c.ymc_cells, c.threads);
+      }
       hipLaunchKernelGGL(bfsk::k_seed<bfsk::FYmc>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FYmc{(gcq::ymc::Queue*)c.qa, (gcq::ymc::Handle*)c.ha}, c.level, src,
+                         bfsk::FYmc{(gcq::ymc::Queue*)c.q[0], (gcq::ymc::Handle*)c.h[0]}, c.level, src,
                          (bfsk::Ctl*)c.ctl);
       break;
     }
@@ -424,7 +464,7 @@ int reset_run(gcq_graph* g, int32_t src) {
 }
 
 template <class F>
-int traverse(gcq_graph* g, F qa, F qb, const gcq_bfs_config* cfg, double* kernel_ms) {
+int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* cfg, double* kernel_ms) {
   BfsCtx& c = g->ctx;
   bfsk::Ctl* ctl = (bfsk::Ctl*)c.ctl;
   const uint32_t grid = cfg->threads / cfg->block;
@@ -442,8 +482,8 @@ int traverse(gcq_graph* g, F qa, F qb, const gcq_bfs_config* cfg, double* kernel
       rc = GCQ_EINVAL;
     } else {
       GCQ_HIP(hipEventRecord(e0, g->stream));
-      hipLaunchKernelGGL(bfsk::k_persistent<F>, dim3(grid), dim3(cfg->block), 0, g->stream, qa, qb, g->d_off,
-                         g->d_idx, c.level, ctl);
+      hipLaunchKernelGGL(bfsk::k_persistent<F>, dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off, g->d_idx,
+                         c.level, ctl);
       GCQ_HIP(hipEventRecord(e1, g->stream));
     }
   } else {
@@ -452,10 +492,9 @@ int traverse(gcq_graph* g, F qa, F qb, const gcq_bfs_config* cfg, double* kernel
     GCQ_HIP(hipEventRecord(e0, g->stream));
     int32_t L = 0;
     for (;; ++L) {
-      const F& qin = (L & 1) ? qb : qa;
-      const F& qout = (L & 1) ? qa : qb;
-      hipLaunchKernelGGL(bfsk::k_level<F>, dim3(grid), dim3(cfg->block), 0, g->stream, qin, qout, g->d_off, g->d_idx,
-                         c.level, L, ctl);
+      hipLaunchKernelGGL(bfsk::k_level<F>, dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off, g->d_idx, c.level,
+                         L, ctl);
+      if (F::kQueues == 3) hipLaunchKernelGGL(bfsk::k_rewind<F>, dim3(1), dim3(64), 0, g->stream, fq, L);
       GCQ_HIP(hipMemcpyAsync(c.pinned, &ctl->next[L & 1], 4, hipMemcpyDeviceToHost, g->stream));
       GCQ_HIP(hipStreamSynchronize(g->stream));
       if (*(volatile uint32_t*)c.pinned == 0) break;
@@ -589,19 +628,30 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   BfsCtx& c = g->ctx;
   double kms = 0;
   switch (c.kind) {
-    case GCQ_GWFQ:
-      rc = traverse(g, bfsk::FGwf{(gcq::gwf::Ring*)c.qa}, bfsk::FGwf{(gcq::gwf::Ring*)c.qb}, cfg, &kms);
+    case GCQ_GWFQ: {
+      bfsk::Frontiers<bfsk::FGwf> fq;
+      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FGwf{(gcq::gwf::Ring*)c.q[i % 2]};
+      rc = traverse(g, fq, cfg, &kms);
       break;
-    case GCQ_GLFQ:
-      rc = traverse(g, bfsk::FLfq{(gcq::lfq::Ring*)c.qa}, bfsk::FLfq{(gcq::lfq::Ring*)c.qb}, cfg, &kms);
+    }
+    case GCQ_GLFQ: {
+      bfsk::Frontiers<bfsk::FLfq> fq;
+      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FLfq{(gcq::lfq::Ring*)c.q[i % 2]};
+      rc = traverse(g, fq, cfg, &kms);
       break;
-    case GCQ_SFQ:
-      rc = traverse(g, bfsk::FSfq{(gcq::sfq::Queue*)c.qa}, bfsk::FSfq{(gcq::sfq::Queue*)c.qb}, cfg, &kms);
+    }
+    case GCQ_SFQ: {
+      bfsk::Frontiers<bfsk::FSfq> fq;
+      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FSfq{(gcq::sfq::Queue*)c.q[i % 2]};
+      rc = traverse(g, fq, cfg, &kms);
       break;
-    case GCQ_WFQ:
-      rc = traverse(g, bfsk::FYmc{(gcq::ymc::Queue*)c.qa, (gcq::ymc::Handle*)c.ha},
-                    bfsk::FYmc{(gcq::ymc::Queue*)c.qb, (gcq::ymc::Handle*)c.hb}, cfg, &kms);
+    }
+    case GCQ_WFQ: {
+      bfsk::Frontiers<bfsk::FYmc> fq;
+      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FYmc{(gcq::ymc::Queue*)c.q[i], (gcq::ymc::Handle*)c.h[i]};
+      rc = traverse(g, fq, cfg, &kms);
       break;
+    }
   }
   if (rc) return rc;
   bfsk::Ctl h;

@@ -352,5 +352,32 @@ __device__ __forceinline__ uint64_t high_water(Queue* q) {
   return t > h ? t : h;
 }
 
+// Recycling a quiescent queue (no operation in flight and none starting
+// until rewind() has completed): every thread of a grid calls
+// recycle_cells to clear its share of the touched cells and reset_handle on
+// its own handle (the owner reads its private fields with plain loads),
+// then one thread calls rewind() once all of them are done.  Shared words
+// are cleared with device-scope stores so waves on other XCDs see them.
+__device__ inline void recycle_cells(Queue* q, uint64_t gid, uint64_t stride) {
+  const uint64_t hw = min(high_water(q), q->g.cells);
+  uint64_t* w = (uint64_t*)cells(q);
+  for (uint64_t i = gid; i < 2 * hw; i += stride) st_rlx(&w[i], 0ull);
+}
+__device__ inline void reset_handle(Queue* q, Handle* hs, uint32_t tid) {
+  const uint32_t n = q->g.nthreads;
+  if (tid >= n) return;
+  Handle* h = &hs[tid];
+  st_rlx(&h->enq_val, 0ull);
+  st_rlx(&h->enq_state, 0ull);
+  st_rlx(&h->deq_id, 0ull);
+  st_rlx(&h->deq_state, 0ull);
+  h->enq_id_last = 0;
+  h->enq_peer = h->deq_peer = (tid + 1) % n;
+}
+__device__ inline void rewind(Queue* q) {
+  st_rlx(Tp(q), 0ull);
+  st_rlx(Hp(q), 0ull);
+}
+
 }  // namespace ymc
 }  // namespace gcq

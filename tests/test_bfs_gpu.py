@@ -89,6 +89,15 @@ def test_lattice_1000_persistent_reuse(gcq, orc, kind):
             _run_and_check(orc, g, src, kind, 8192, 1, off, idx)
 
 
+@pytest.mark.parametrize("mode", [1, 0], ids=[MODES[m] for m in (1, 0)])
+def test_wfq_cells_recycled_65536_threads(gcq, orc, mode):
+    # ~2,000 levels x 65,536 lanes would burn ~10^8 WFQ cells without the
+    # three-frontier recycling; the pool holds V + 4 T + 2^20 cells.
+    from gcq_amd import bfs
+    with bfs.Graph.lattice(1000, 1000, 0.75, 0.01, 7) as g:
+        _run_and_check(orc, g, 0, 2, 65536, mode)
+
+
 def test_config3_lattice_4900(gcq, orc):
     # The benchmark graph: 4900 x 4900 lattice, p = 0.75, 1% diagonals,
     # seed 7, source 0, 8,192 threads (block 256), G-LFQ, persistent.
