@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 from . import GLFQ, GWFQ, SFQ, WFQ, _check, load  # noqa: F401
-from .workloads import BfsConfig, BfsResult
+from .workloads import BatchConfig, BatchResult, BfsConfig, BfsResult, QueryResult, TuneTrial
 
 HOST_LOOP, PERSISTENT = 0, 1
 
@@ -87,6 +87,76 @@ class Graph:
             pass
 
 
+def batch_config(**kw):
+    """The library's default batch configuration with keyword overrides."""
+    c = BatchConfig()
+    load().gcq_batch_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _cfg_dict(c):
+    return {f: getattr(c, f) for f, _ in BatchConfig._fields_ if f != "pad"}
+
+
+class Batch:
+    """Batched BFS over resident graphs: many queries in one persistent
+    kernel through one shared frontier queue."""
+
+    def __init__(self, graphs):
+        self.graphs = list(graphs)  # keep the handles alive
+        arr = (C.c_void_p * len(self.graphs))(*[g.handle for g in self.graphs])
+        h = C.c_void_p()
+        _check(load().gcq_batch_create(arr, len(self.graphs), C.byref(h)), "gcq_batch_create")
+        self._h = h
+
+    def run(self, graph_ids, sources, cfg=None, depths=False, **kw):
+        """Returns (list of depth arrays or None, per-query dicts, batch dict)."""
+        gids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        srcs = np.ascontiguousarray(sources, dtype=np.int32)
+        nq = len(gids)
+        cfg = cfg if cfg is not None else batch_config(**kw)
+        outs, ptrs = None, None
+        if depths:
+            outs = [np.empty(self.graphs[g].info()[0], dtype=np.int32) for g in gids]
+            ptrs = (C.c_void_p * nq)(*[o.ctypes.data for o in outs])
+        pq = (QueryResult * nq)()
+        res = BatchResult()
+        _check(load().gcq_batch_run(self._h, C.byref(cfg), nq, gids.ctypes.data, srcs.ctypes.data, ptrs, pq,
+                                    C.byref(res)), "gcq_batch_run")
+        per = [{f: getattr(q, f) for f, _ in QueryResult._fields_} for q in pq]
+        return outs, per, {f: getattr(res, f) for f, _ in BatchResult._fields_}
+
+    def autotune(self, graph_ids, sources, kind=GLFQ, reps=3, max_trials=256):
+        gids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        srcs = np.ascontiguousarray(sources, dtype=np.int32)
+        best = BatchConfig()
+        trials = (TuneTrial * max_trials)()
+        nt = C.c_int32(0)
+        _check(load().gcq_batch_autotune(self._h, kind, len(gids), gids.ctypes.data, srcs.ctypes.data, reps,
+                                         C.byref(best), trials, max_trials, C.byref(nt)), "gcq_batch_autotune")
+        rows = [dict(_cfg_dict(t.cfg), ms=t.ms) for t in trials[: min(nt.value, max_trials)]]
+        return best, rows
+
+    def close(self):
+        if self._h is not None:
+            load().gcq_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+
 def lattice(rows, cols, keep=0.75, diag=0.01, seed=7):
     """Host CSR (offsets, indices) of the synthetic lattice."""
     with Graph.lattice(rows, cols, keep, diag, seed) as g:
@@ -97,3 +167,16 @@ def bfs_depths(offsets, indices, src, kind=GLFQ, threads=1024, block=256, mode=P
     with Graph.from_csr(offsets, indices) as g:
         d, _ = g.bfs(src, kind=kind, threads=threads, block=block, mode=mode)
         return d
+
+
+def rmat_query_set(nq=256, scale_lo=12, scale_hi=20, edge_factor=16, seed=11):
+    """The batched-BFS workload: R-MAT (a, b, c) = (.57, .19, .19) graphs of
+    scales scale_lo..scale_hi (one resident graph per scale), nq queries whose
+    scale is drawn uniformly (so V is log-uniform) with a random source of
+    nonzero degree.  Returns (graphs, graph_ids, sources)."""
+    scales = list(range(scale_lo, scale_hi + 1))
+    graphs = [Graph.rmat(s, edge_factor, 0.57, 0.19, 0.19, seed) for s in scales]
+    rng = np.random.default_rng(seed)
+    gids = rng.integers(0, len(scales), nq).astype(np.int32)
+    srcs = np.array([graphs[g].pick_source(seed, i) for i, g in enumerate(gids)], dtype=np.int32)
+    return graphs, gids, srcs

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libgcq.so")
 ARCH = os.environ.get("GCQ_OFFLOAD_ARCH", "gfx950")
 
 # Every HIP TU of the library.
-SOURCES = ["core.hip", "bfs.hip", "graph.cpp", "rt.hip", "serve.hip"]
+SOURCES = ["core.hip", "bfs.hip", "batch.hip", "graph.cpp", "rt.hip", "serve.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -29,143 +29,28 @@
 #include <cstring>
 #include <vector>
 
-#include "graph.hpp"
-#include "queue_impl.hpp"
+#include <type_traits>
+
+#include "graph_handle.hpp"
 
 // ======================================================================== graph handle
-struct BfsCtx {
-  int kind = -1;
-  uint32_t threads = 0, capacity = 0;
-  uint64_t ymc_cells = 0;
-  int nq = 0;                          // frontier queues (2, or 3 for WFQ)
-  void* q[3] = {nullptr, nullptr, nullptr};
-  void* h[3] = {nullptr, nullptr, nullptr};  // WFQ handles
-  int32_t* level = nullptr;
-  void* ctl = nullptr;
-  uint32_t* pinned = nullptr;  // host-visible counters for the host-loop mode
-};
-
-struct gcq_graph {
-  gcqg::Csr host;
-  int32_t* d_off = nullptr;
-  int32_t* d_idx = nullptr;
-  hipStream_t stream = nullptr;
-  BfsCtx ctx;
-};
-
 namespace bfsk {
 
-using gcq::lane_id;
+using namespace gcqf;
 
-// Control block: each contended word on its own 128-B line.
+// Control block: the barrier, then each contended counter on its own 128-B line.
 struct Ctl {
-  uint32_t bar_count, p0[31];
-  uint32_t bar_gen, p1[31];
+  Barrier bar;
   uint32_t next[3], p2[29];
-  uint32_t abort, err, p3[30];
+  uint32_t err, p3[31];
   uint64_t edges, reached, p4[14];
   int32_t levels, p5[31];
 };
-constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
-
-// ---------------------------------------------------------------- frontier adapters
-struct FGwf {
-  static constexpr int kQueues = 2;
-  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
-  __device__ __forceinline__ void rewind() const {}
-  gcq::gwf::Ring* r;
-  __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
-    gcq::gwf::NoProbe np;
-    gcq::gwf::RingOps<> o(r, tid, &np);
-    o.enqueue(v);
-    return true;
-  }
-  __device__ __forceinline__ bool pop(uint32_t tid, uint32_t* v) const {
-    gcq::gwf::NoProbe np;
-    gcq::gwf::RingOps<> o(r, tid, &np);
-    const uint32_t x = o.dequeue(true);
-    if (x == gcq::gwf::kBot) return false;
-    *v = x;
-    return true;
-  }
-};
-struct FLfq {
-  static constexpr int kQueues = 2;
-  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
-  __device__ __forceinline__ void rewind() const {}
-  gcq::lfq::Ring* r;
-  __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
-    gcq::lfq::enqueue(r, v);
-    return true;
-  }
-  __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
-    const uint32_t x = gcq::lfq::dequeue(r, (gcq::lfq::NoProbe*)nullptr, true);
-    if (x == gcq::lfq::kBot) return false;
-    *v = x;
-    return true;
-  }
-};
-struct FSfq {
-  static constexpr int kQueues = 2;
-  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
-  __device__ __forceinline__ void rewind() const {}
-  gcq::sfq::Queue* q;
-  __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
-    return gcq::sfq::enqueue_blocking(q, (uint64_t)v, 1ull << 22) == gcq::sfq::kSuccess;
-  }
-  __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
-    uint64_t x = 0;
-    if (!gcq::sfq::dequeue_or_empty(q, &x, 1ull << 20)) return false;
-    *v = (uint32_t)x;
-    return true;
-  }
-};
-// WFQ cells are never reused by the algorithm, and every level burns up to one
-// cell per idle lane, so BFS rotates three WFQ frontiers: while one is the
-// level's input and one its output, the third (drained a level earlier)
-// is cleared by the grid and rewound at the level's barrier.
-struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt.hpp:42-46)
-  static constexpr int kQueues = 3;
-  gcq::ymc::Queue* q;
-  gcq::ymc::Handle* h;
-  __device__ __forceinline__ void recycle(uint64_t gid, uint64_t stride, uint32_t tid) const {
-    gcq::ymc::recycle_cells(q, gid, stride);
-    gcq::ymc::reset_handle(q, h, tid);
-  }
-  __device__ __forceinline__ void rewind() const { gcq::ymc::rewind(q); }
-  __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
-    return gcq::ymc::enqueue(q, h, tid, (uint64_t)v + 1);
-  }
-  __device__ __forceinline__ bool pop(uint32_t tid, uint32_t* v) const {
-    const uint64_t x = gcq::ymc::dequeue(q, h, tid);
-    if (x == gcq::ymc::kEmpty) return false;
-    *v = (uint32_t)(x - 1);
-    return true;
-  }
-};
-
-__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 struct LevelAcc {
   uint64_t edges;
   uint32_t disc;
 };
-
-// Enqueue the first n staged vertices (n <= 64) and shift the rest down.
-template <class F>
-__device__ __forceinline__ void flush(const F& qout, uint32_t tid, uint32_t* stage, uint32_t& sc, uint32_t n,
-                                      uint32_t* err) {
-  const uint32_t lane = lane_id();
-  const uint32_t v = stage[lane];
-  if (lane < n && !qout.push(tid, v)) gcq::or_rlx(err, 1u);
-  const uint32_t rest = sc - n;
-  const uint32_t mv = lane < rest ? stage[64 + lane] : 0u;
-  if (lane < rest) stage[lane] = mv;
-  sc = rest;
-}
 
 // Drain qin (level L) and enqueue the vertices discovered at depth L+1.
 // All 64 lanes of the wave must call it together (threads % 64 == 0).
@@ -223,38 +108,6 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
   if (sc) flush(qout, tid, stage, sc, sc, err);
 }
 
-// Sense-reversing grid barrier over co-resident blocks.  The last block to
-// arrive runs `last()` before releasing the others.  Returns false once any
-// block has aborted (spin limit), so every block leaves its loop.
-template <class Last>
-__device__ bool grid_barrier(Ctl* c, uint32_t nblocks, Last last) {
-  gcq::vm_wait();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t g = gcq::ld_rlx(&c->bar_gen);
-    gcq::vm_wait();  // generation read before arriving
-    const uint32_t a = gcq::faa_rlx(&c->bar_count, 1u);
-    if (a == nblocks - 1) {
-      last();
-      gcq::st_rlx(&c->bar_count, 0u);
-      gcq::vm_wait();
-      gcq::st_rlx(&c->bar_gen, g + 1u);
-    } else {
-      uint64_t spins = 0;
-      while (gcq::ld_rlx(&c->bar_gen) == g) {
-        if (gcq::ld_rlx(&c->abort)) break;
-        if (++spins > kBarrierSpinLimit) {
-          gcq::st_rlx(&c->abort, 1u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-  }
-  __syncthreads();
-  return gcq::ld_rlx(&c->abort) == 0u;
-}
-
 template <class F>
 __device__ void finish_counts(LevelAcc& acc, uint64_t disc_total, Ctl* c) {
   const uint64_t e = gcq::wave_sum(acc.edges);
@@ -264,16 +117,6 @@ __device__ void finish_counts(LevelAcc& acc, uint64_t disc_total, Ctl* c) {
     if (d) gcq::faa_rlx(&c->reached, d);
   }
 }
-
-// Frontier queues of one traversal: level L reads q[L % K] and writes
-// q[(L + 1) % K]; with K = 3 the third one is recycled during the level.
-template <class F>
-struct Frontiers {
-  F q[3];
-  __device__ __forceinline__ const F& in(int32_t L) const { return q[L % F::kQueues]; }
-  __device__ __forceinline__ const F& out(int32_t L) const { return q[(L + 1) % F::kQueues]; }
-  __device__ __forceinline__ const F& idle(int32_t L) const { return q[(L + 2) % 3]; }
-};
 
 template <class F>
 __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
@@ -292,7 +135,7 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
     disc_total += acc.disc;
     const uint32_t d = wave_sum32(acc.disc);
     if (lane_id() == 0 && d) gcq::faa_rlx(&c->next[L % 3], d);
-    if (!grid_barrier(c, gridDim.x, [&]() {
+    if (!grid_barrier(&c->bar, gridDim.x, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
         }))
       break;
@@ -324,34 +167,6 @@ __global__ void k_rewind(Frontiers<F> fq, int32_t L) {
   if (threadIdx.x == 0) fq.idle(L).rewind();
 }
 
-// ---------------------------------------------------------------- resets
-__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = gid; i < n; i += st) p[i] = v;
-}
-__global__ void k_zero_u64(uint64_t* p, uint64_t n) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = gid; i < n; i += st) p[i] = 0;
-}
-__global__ void k_reset_gwf(void* a, void* b, gcq::gwf::InitParams p) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
-  gcq::gwf::init_slice((gcq::gwf::Ring*)a, p, false, gid, st);
-  gcq::gwf::init_slice((gcq::gwf::Ring*)b, p, false, gid, st);
-}
-__global__ void k_reset_lfq(void* a, void* b, uint32_t n) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
-  gcq::lfq::init_slice((gcq::lfq::Ring*)a, n, false, 0, gid, st, 0);
-  gcq::lfq::init_slice((gcq::lfq::Ring*)b, n, false, 0, gid, st, 0);
-}
-__global__ void k_reset_sfq(void* a, void* b, uint64_t n) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
-  gcq::sfq::init_slice((gcq::sfq::Queue*)a, n, 0, gid, st);
-  gcq::sfq::init_slice((gcq::sfq::Queue*)b, n, 0, gid, st);
-}
-__global__ void k_reset_ymc(void* q, void* h, uint64_t cells, uint32_t nthreads) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
-  gcq::ymc::init_slice((gcq::ymc::Queue*)q, (gcq::ymc::Handle*)h, cells, nthreads, 16, gid, st);
-}
 // Seed the level-0 frontier with the source.
 template <class F>
 __global__ void k_seed(F q, int32_t* level, int32_t src, Ctl* c) {
@@ -366,16 +181,11 @@ __global__ void k_seed(F q, int32_t* level, int32_t src, Ctl* c) {
 // ======================================================================== host side
 namespace {
 
-uint32_t pow2_at_least(uint64_t x) {
-  uint64_t p = 64;
-  while (p < x) p <<= 1;
-  return (uint32_t)std::min<uint64_t>(p, 1ull << 31);
-}
-
 int ctx_free(gcq_graph* g) {
   BfsCtx& c = g->ctx;
-  for (void* p : {c.q[0], c.q[1], c.q[2], c.h[0], c.h[1], c.h[2], (void*)c.level, c.ctl})
-    if (p) (void)hipFree(p);
+  c.fs.release();
+  if (c.level) (void)hipFree(c.level);
+  if (c.ctl) (void)hipFree(c.ctl);
   if (c.pinned) (void)hipHostFree(c.pinned);
   c = BfsCtx();
   return GCQ_OK;
@@ -383,82 +193,29 @@ int ctx_free(gcq_graph* g) {
 
 int ctx_ensure(gcq_graph* g, int kind, uint32_t threads) {
   BfsCtx& c = g->ctx;
-  if (c.kind == kind && c.threads >= threads && c.level) return GCQ_OK;
-  ctx_free(g);
   const uint32_t nv = (uint32_t)g->host.nv;
-  uint64_t qbytes = 0;
-  c.capacity = pow2_at_least(std::max<uint64_t>(nv, threads));
-  switch (kind) {
-    case GCQ_GWFQ:
-      GCQ_REQUIRE(c.capacity <= (1u << 25), "G-WFQ frontier capacity %u exceeds 2^25 (V too large)", c.capacity);
-      qbytes = gcq::gwf::ring_bytes(c.capacity, threads);
-      break;
-    case GCQ_GLFQ: qbytes = gcq::lfq::ring_bytes(c.capacity); break;
-    case GCQ_SFQ: qbytes = gcq::sfq::queue_bytes(c.capacity); break;
-    case GCQ_WFQ:
-      // One level's cells: its frontier (<= V) plus at most one burned cell
-      // per dequeuing lane; the queue is recycled two levels later.
-      c.ymc_cells = (uint64_t)nv + 4ull * threads + (1ull << 20);
-      qbytes = gcq::ymc::queue_bytes(c.ymc_cells);
-      break;
-    default: GCQ_REQUIRE(false, "unknown queue kind %d", kind);
+  int rc = c.fs.ensure(kind, nv, threads);
+  if (rc) return rc;
+  if (!c.level) {
+    GCQ_HIP(hipMalloc(&c.level, 4ull * nv));
+    GCQ_HIP(hipMalloc(&c.ctl, sizeof(bfsk::Ctl)));
+    GCQ_HIP(hipHostMalloc((void**)&c.pinned, 64));
   }
-  c.nq = kind == GCQ_WFQ ? 3 : 2;
-  for (int i = 0; i < c.nq; ++i) {
-    GCQ_HIP(hipMalloc(&c.q[i], qbytes));
-    if (kind == GCQ_WFQ) GCQ_HIP(hipMalloc(&c.h[i], sizeof(gcq::ymc::Handle) * (size_t)threads));
-  }
-  GCQ_HIP(hipMalloc(&c.level, 4ull * nv));
-  GCQ_HIP(hipMalloc(&c.ctl, sizeof(bfsk::Ctl)));
-  GCQ_HIP(hipHostMalloc((void**)&c.pinned, 64));
-  c.kind = kind;
-  c.threads = threads;
   return GCQ_OK;
 }
-
-uint32_t grid_of(uint64_t work) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + 255) / 256, 8192)); }
 
 int reset_run(gcq_graph* g, int32_t src) {
   BfsCtx& c = g->ctx;
   const uint32_t nv = (uint32_t)g->host.nv;
-  hipLaunchKernelGGL(bfsk::k_fill_i32, dim3(grid_of(nv)), dim3(256), 0, g->stream, c.level, (uint64_t)nv, -1);
-  hipLaunchKernelGGL(bfsk::k_zero_u64, dim3(1), dim3(256), 0, g->stream, (uint64_t*)c.ctl,
+  hipLaunchKernelGGL(gcqf::kf_fill_i32, dim3(gcqf::grid_of(nv)), dim3(256), 0, g->stream, c.level, (uint64_t)nv, -1);
+  hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, g->stream, (uint64_t*)c.ctl,
                      (uint64_t)(sizeof(bfsk::Ctl) / 8));
-  switch (c.kind) {
-    case GCQ_GWFQ: {
-      gcq::gwf::InitParams p{c.capacity, c.threads, 0, 64, 128, 256, 0};
-      hipLaunchKernelGGL(bfsk::k_reset_gwf, dim3(grid_of(2ull * c.capacity)), dim3(256), 0, g->stream, c.q[0], c.q[1],
-                         p);
-      hipLaunchKernelGGL(bfsk::k_seed<bfsk::FGwf>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FGwf{(gcq::gwf::Ring*)c.q[0]}, c.level, src, (bfsk::Ctl*)c.ctl);
-      break;
-    }
-    case GCQ_GLFQ:
-      hipLaunchKernelGGL(bfsk::k_reset_lfq, dim3(grid_of(2ull * c.capacity)), dim3(256), 0, g->stream, c.q[0], c.q[1],
-                         c.capacity);
-      hipLaunchKernelGGL(bfsk::k_seed<bfsk::FLfq>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FLfq{(gcq::lfq::Ring*)c.q[0]}, c.level, src, (bfsk::Ctl*)c.ctl);
-      break;
-    case GCQ_SFQ:
-      hipLaunchKernelGGL(bfsk::k_reset_sfq, dim3(grid_of(c.capacity)), dim3(256), 0, g->stream, c.q[0], c.q[1],
-                         (uint64_t)c.capacity);
-      hipLaunchKernelGGL(bfsk::k_seed<bfsk::FSfq>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FSfq{(gcq::sfq::Queue*)c.q[0]}, c.level, src, (bfsk::Ctl*)c.ctl);
-      break;
-    case GCQ_WFQ: {
-      const uint64_t words = c.ymc_cells * 2;
-      for (int i = 0; i < 3; ++i) {
-        hipLaunchKernelGGL(bfsk::k_zero_u64, dim3(grid_of(words)), dim3(256), 0, g->stream,
-                           (uint64_t*)((char*)c.q[i] + gcq::ymc::kHdrBytes), words);
-        hipLaunchKernelGGL(bfsk::k_reset_ymc, dim3(grid_of(c.threads)), dim3(256), 0, g->stream, c.q[i], c.h[i],
-                           c.ymc_cells, c.threads);
-      }
-      hipLaunchKernelGGL(bfsk::k_seed<bfsk::FYmc>, dim3(1), dim3(64), 0, g->stream,
-                         bfsk::FYmc{(gcq::ymc::Queue*)c.q[0], (gcq::ymc::Handle*)c.h[0]}, c.level, src,
-                         (bfsk::Ctl*)c.ctl);
-      break;
-    }
-  }
+  c.fs.reset(g->stream);
+  c.fs.dispatch([&](const auto& fq) {
+    using F = std::decay_t<decltype(fq.q[0])>;
+    hipLaunchKernelGGL(bfsk::k_seed<F>, dim3(1), dim3(64), 0, g->stream, fq.q[0], c.level, src, (bfsk::Ctl*)c.ctl);
+    return GCQ_OK;
+  });
   GCQ_HIP(hipGetLastError());
   return GCQ_OK;
 }
@@ -513,9 +270,14 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
   return rc;
 }
 
-// Copy the CSR to HBM on first use (graphs can be built and inspected on a
-// host without a GPU).
-int ensure_uploaded(gcq_graph* g) {
+int finish_create(gcq_graph* g, gcq_graph** out) {
+  *out = g;
+  return GCQ_OK;
+}
+
+}  // namespace
+
+int gcqg::ensure_uploaded(gcq_graph* g) {
   if (g->d_off) return GCQ_OK;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
@@ -530,13 +292,6 @@ int ensure_uploaded(gcq_graph* g) {
   if (ne) GCQ_HIP(hipMemcpy(g->d_idx, g->host.indices.data(), 4ull * ne, hipMemcpyHostToDevice));
   return GCQ_OK;
 }
-
-int finish_create(gcq_graph* g, gcq_graph** out) {
-  *out = g;
-  return GCQ_OK;
-}
-
-}  // namespace
 
 extern "C" int gcq_graph_lattice(int32_t rows, int32_t cols, double keep, double diag, uint64_t seed,
                                  gcq_graph** out) {
@@ -618,7 +373,7 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   GCQ_REQUIRE(cfg->mode == GCQ_BFS_HOST_LOOP || cfg->mode == GCQ_BFS_PERSISTENT, "bad mode");
   std::memset(res, 0, sizeof *res);
   if (src < 0 || src >= g->host.nv) src = 0;  // bfs_bench.cpp:555
-  int rc = ensure_uploaded(g);
+  int rc = gcqg::ensure_uploaded(g);
   if (rc) return rc;
   const auto t0 = std::chrono::steady_clock::now();
   rc = ctx_ensure(g, cfg->kind, cfg->threads);
@@ -627,32 +382,7 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   if (rc) return rc;
   BfsCtx& c = g->ctx;
   double kms = 0;
-  switch (c.kind) {
-    case GCQ_GWFQ: {
-      bfsk::Frontiers<bfsk::FGwf> fq;
-      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FGwf{(gcq::gwf::Ring*)c.q[i % 2]};
-      rc = traverse(g, fq, cfg, &kms);
-      break;
-    }
-    case GCQ_GLFQ: {
-      bfsk::Frontiers<bfsk::FLfq> fq;
-      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FLfq{(gcq::lfq::Ring*)c.q[i % 2]};
-      rc = traverse(g, fq, cfg, &kms);
-      break;
-    }
-    case GCQ_SFQ: {
-      bfsk::Frontiers<bfsk::FSfq> fq;
-      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FSfq{(gcq::sfq::Queue*)c.q[i % 2]};
-      rc = traverse(g, fq, cfg, &kms);
-      break;
-    }
-    case GCQ_WFQ: {
-      bfsk::Frontiers<bfsk::FYmc> fq;
-      for (int i = 0; i < 3; ++i) fq.q[i] = bfsk::FYmc{(gcq::ymc::Queue*)c.q[i], (gcq::ymc::Handle*)c.h[i]};
-      rc = traverse(g, fq, cfg, &kms);
-      break;
-    }
-  }
+  rc = c.fs.dispatch([&](const auto& fq) { return traverse(g, fq, cfg, &kms); });
   if (rc) return rc;
   bfsk::Ctl h;
   GCQ_HIP(hipMemcpyAsync(&h, c.ctl, sizeof h, hipMemcpyDeviceToHost, g->stream));
@@ -664,7 +394,7 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   res->levels = h.levels;
   res->kernel_ms = kms;
   res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  if (h.abort) {
+  if (h.bar.abort) {
     gcqh::set_error("BFS aborted: a grid-barrier wait exceeded its spin limit");
     return GCQ_ETIMEOUT;
   }

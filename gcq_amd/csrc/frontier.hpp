@@ -1,0 +1,315 @@
+// frontier.hpp — frontier queues for the BFS kernels (single, batched and
+// served), shared by bfs.hip and batch.hip.
+//
+// A frontier adapter wraps one queue of any of the four kinds and carries a
+// 32-bit payload in the queue itself (G-WFQ / G-LFQ ring indices, SFQ items,
+// WFQ values + 1): one queue operation per frontier vertex instead of the
+// reference's data-array indirection (bfs_bench.cpp:35-132 goes through
+// Enqueue_Ptr / fq_enqueue_ptr: two ring operations and a fence).
+// Capacity is sized by the host to the largest frontier, so pushes do not
+// fail (the reference ignores failed pushes, bfs_bench.cpp:380, 404, 475).
+#pragma once
+#include <algorithm>
+
+#include "queue_impl.hpp"
+
+namespace gcqf {
+
+using gcq::lane_id;
+
+constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
+
+// ---------------------------------------------------------------- adapters
+struct FGwf {
+  static constexpr int kQueues = 2;
+  gcq::gwf::Ring* r;
+  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ void rewind() const {}
+  __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
+    gcq::gwf::NoProbe np;
+    gcq::gwf::RingOps<> o(r, tid, &np);
+    o.enqueue(v);
+    return true;
+  }
+  __device__ __forceinline__ bool pop(uint32_t tid, uint32_t* v) const {
+    gcq::gwf::NoProbe np;
+    gcq::gwf::RingOps<> o(r, tid, &np);
+    const uint32_t x = o.dequeue(true);
+    if (x == gcq::gwf::kBot) return false;
+    *v = x;
+    return true;
+  }
+};
+struct FLfq {
+  static constexpr int kQueues = 2;
+  gcq::lfq::Ring* r;
+  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ void rewind() const {}
+  __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
+    gcq::lfq::enqueue(r, v);
+    return true;
+  }
+  __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
+    const uint32_t x = gcq::lfq::dequeue(r, (gcq::lfq::NoProbe*)nullptr, true);
+    if (x == gcq::lfq::kBot) return false;
+    *v = x;
+    return true;
+  }
+};
+struct FSfq {
+  static constexpr int kQueues = 2;
+  gcq::sfq::Queue* q;
+  __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ void rewind() const {}
+  __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
+    return gcq::sfq::enqueue_blocking(q, (uint64_t)v, 1ull << 22) == gcq::sfq::kSuccess;
+  }
+  __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
+    uint64_t x = 0;
+    if (!gcq::sfq::dequeue_or_empty(q, &x, 1ull << 20)) return false;
+    *v = (uint32_t)x;
+    return true;
+  }
+};
+// WFQ cells are never reused by the algorithm, and every level burns up to one
+// cell per idle lane, so the kernels rotate three WFQ frontiers: while one is
+// the level's input and one its output, the third (drained a level earlier)
+// is cleared by the grid and rewound at the level's barrier.
+struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt.hpp:42-46)
+  static constexpr int kQueues = 3;
+  gcq::ymc::Queue* q;
+  gcq::ymc::Handle* h;
+  __device__ __forceinline__ void recycle(uint64_t gid, uint64_t stride, uint32_t tid) const {
+    gcq::ymc::recycle_cells(q, gid, stride);
+    gcq::ymc::reset_handle(q, h, tid);
+  }
+  __device__ __forceinline__ void rewind() const { gcq::ymc::rewind(q); }
+  __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
+    return gcq::ymc::enqueue(q, h, tid, (uint64_t)v + 1);
+  }
+  __device__ __forceinline__ bool pop(uint32_t tid, uint32_t* v) const {
+    const uint64_t x = gcq::ymc::dequeue(q, h, tid);
+    if (x == gcq::ymc::kEmpty) return false;
+    *v = (uint32_t)(x - 1);
+    return true;
+  }
+};
+
+// Frontier queues of one traversal: level L reads q[L % K] and writes
+// q[(L + 1) % K]; with K = 3 the third one is recycled during the level.
+template <class F>
+struct Frontiers {
+  F q[3];
+  __device__ __forceinline__ const F& in(int32_t L) const { return q[L % F::kQueues]; }
+  __device__ __forceinline__ const F& out(int32_t L) const { return q[(L + 1) % F::kQueues]; }
+  __device__ __forceinline__ const F& idle(int32_t L) const { return q[(L + 2) % 3]; }
+};
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Enqueue the first n staged items (n <= 64) of a wave's LDS stage and
+// shift the rest down; `sc` (wave-uniform) is the staged count.
+template <class F>
+__device__ __forceinline__ void flush(const F& qout, uint32_t tid, uint32_t* stage, uint32_t& sc, uint32_t n,
+                                      uint32_t* err) {
+  const uint32_t lane = lane_id();
+  const uint32_t v = stage[lane];
+  if (lane < n && !qout.push(tid, v)) gcq::or_rlx(err, 1u);
+  const uint32_t rest = sc - n;
+  const uint32_t mv = lane < rest ? stage[64 + lane] : 0u;
+  if (lane < rest) stage[lane] = mv;
+  sc = rest;
+}
+
+// Barrier words, each on its own 128-B line.
+struct Barrier {
+  uint32_t count, p0[31];
+  uint32_t gen, p1[31];
+  uint32_t abort, p2[31];
+};
+
+// Sense-reversing grid barrier over co-resident blocks.  The last block to
+// arrive runs `last()` before releasing the others.  Returns false once any
+// block has aborted (spin limit), so every block leaves its loop.
+template <class Last>
+__device__ bool grid_barrier(Barrier* b, uint32_t nblocks, Last last) {
+  gcq::vm_wait();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t g = gcq::ld_rlx(&b->gen);
+    gcq::vm_wait();  // generation read before arriving
+    const uint32_t a = gcq::faa_rlx(&b->count, 1u);
+    if (a == nblocks - 1) {
+      last();
+      gcq::st_rlx(&b->count, 0u);
+      gcq::vm_wait();
+      gcq::st_rlx(&b->gen, g + 1u);
+    } else {
+      uint64_t spins = 0;
+      while (gcq::ld_rlx(&b->gen) == g) {
+        if (gcq::ld_rlx(&b->abort)) break;
+        if (++spins > kBarrierSpinLimit) {
+          gcq::st_rlx(&b->abort, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+  }
+  __syncthreads();
+  return gcq::ld_rlx(&b->abort) == 0u;
+}
+
+// ---------------------------------------------------------------- reset kernels
+namespace {
+__global__ void kf_fill_i32(int32_t* p, uint64_t n, int32_t v) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = gid; i < n; i += st) p[i] = v;
+}
+__global__ void kf_zero_u64(uint64_t* p, uint64_t n) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = gid; i < n; i += st) p[i] = 0;
+}
+__global__ void kf_reset_gwf(void* a, void* b, gcq::gwf::InitParams p) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  gcq::gwf::init_slice((gcq::gwf::Ring*)a, p, false, gid, st);
+  gcq::gwf::init_slice((gcq::gwf::Ring*)b, p, false, gid, st);
+}
+__global__ void kf_reset_lfq(void* a, void* b, uint32_t n) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  gcq::lfq::init_slice((gcq::lfq::Ring*)a, n, false, 0, gid, st, 0);
+  gcq::lfq::init_slice((gcq::lfq::Ring*)b, n, false, 0, gid, st, 0);
+}
+__global__ void kf_reset_sfq(void* a, void* b, uint64_t n) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  gcq::sfq::init_slice((gcq::sfq::Queue*)a, n, 0, gid, st);
+  gcq::sfq::init_slice((gcq::sfq::Queue*)b, n, 0, gid, st);
+}
+__global__ void kf_reset_ymc(void* q, void* h, uint64_t cells, uint32_t nthreads) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  gcq::ymc::init_slice((gcq::ymc::Queue*)q, (gcq::ymc::Handle*)h, cells, nthreads, 16, gid, st);
+}
+}  // namespace
+
+inline uint32_t grid_of(uint64_t work) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + 255) / 256, 8192));
+}
+inline uint32_t pow2_at_least(uint64_t x) {
+  uint64_t p = 64;
+  while (p < x) p <<= 1;
+  return (uint32_t)std::min<uint64_t>(p, 1ull << 31);
+}
+
+// ---------------------------------------------------------------- host side
+// The frontier queues of one traversal context, in HBM.
+struct FrontierSet {
+  int kind = -1, nq = 0;
+  uint32_t threads = 0, capacity = 0;
+  uint64_t ymc_cells = 0;
+  void* q[3] = {nullptr, nullptr, nullptr};
+  void* h[3] = {nullptr, nullptr, nullptr};  // WFQ handles
+
+  void release() {
+    for (int i = 0; i < 3; ++i) {
+      if (q[i]) (void)hipFree(q[i]);
+      if (h[i]) (void)hipFree(h[i]);
+    }
+    *this = FrontierSet();
+  }
+
+  // Queues for up to `items` frontier entries per level, used by at most
+  // `nthreads` threads.  Reuses the allocation when it is large enough.
+  int ensure(int k, uint64_t items, uint32_t nthreads) {
+    const uint32_t cap = pow2_at_least(std::max<uint64_t>(items, nthreads));
+    if (k == kind && nthreads <= threads && cap <= capacity && q[0]) return GCQ_OK;
+    release();
+    uint64_t qbytes = 0;
+    switch (k) {
+      case GCQ_GWFQ:
+        GCQ_REQUIRE(cap <= (1u << 25), "G-WFQ frontier capacity %u exceeds 2^25 entries", cap);
+        qbytes = gcq::gwf::ring_bytes(cap, nthreads);
+        break;
+      case GCQ_GLFQ: qbytes = gcq::lfq::ring_bytes(cap); break;
+      case GCQ_SFQ: qbytes = gcq::sfq::queue_bytes(cap); break;
+      case GCQ_WFQ:
+        // One level's cells: its frontier plus at most one burned cell per
+        // dequeuing lane; the queue is recycled two levels later.
+        ymc_cells = (uint64_t)cap + 4ull * nthreads + (1ull << 20);
+        qbytes = gcq::ymc::queue_bytes(ymc_cells);
+        break;
+      default: GCQ_REQUIRE(false, "unknown queue kind %d", k);
+    }
+    nq = k == GCQ_WFQ ? 3 : 2;
+    for (int i = 0; i < nq; ++i) {
+      if (hipMalloc(&q[i], qbytes) != hipSuccess ||
+          (k == GCQ_WFQ && hipMalloc(&h[i], sizeof(gcq::ymc::Handle) * (size_t)nthreads) != hipSuccess)) {
+        release();
+        gcqh::set_error("out of device memory for frontier queues (%llu bytes each)", (unsigned long long)qbytes);
+        return GCQ_ENOMEM;
+      }
+    }
+    kind = k;
+    threads = nthreads;
+    capacity = cap;
+    return GCQ_OK;
+  }
+
+  // Empty every queue (stream-ordered kernels; no memset nodes, so the
+  // sequence may be captured into a graph).
+  void reset(hipStream_t s) const {
+    switch (kind) {
+      case GCQ_GWFQ: {
+        gcq::gwf::InitParams p{capacity, threads, 0, 64, 128, 256, 0};
+        hipLaunchKernelGGL(kf_reset_gwf, dim3(grid_of(2ull * capacity)), dim3(256), 0, s, q[0], q[1], p);
+        break;
+      }
+      case GCQ_GLFQ:
+        hipLaunchKernelGGL(kf_reset_lfq, dim3(grid_of(2ull * capacity)), dim3(256), 0, s, q[0], q[1], capacity);
+        break;
+      case GCQ_SFQ:
+        hipLaunchKernelGGL(kf_reset_sfq, dim3(grid_of(capacity)), dim3(256), 0, s, q[0], q[1], (uint64_t)capacity);
+        break;
+      case GCQ_WFQ:
+        for (int i = 0; i < 3; ++i) {
+          hipLaunchKernelGGL(kf_zero_u64, dim3(grid_of(ymc_cells * 2)), dim3(256), 0, s,
+                             (uint64_t*)((char*)q[i] + gcq::ymc::kHdrBytes), ymc_cells * 2);
+          hipLaunchKernelGGL(kf_reset_ymc, dim3(grid_of(threads)), dim3(256), 0, s, q[i], h[i], ymc_cells, threads);
+        }
+        break;
+    }
+  }
+
+  // Call fn(Frontiers<F>) with the adapter type of this set's kind.
+  template <class Fn>
+  int dispatch(Fn&& fn) const {
+    switch (kind) {
+      case GCQ_GWFQ: {
+        Frontiers<FGwf> f;
+        for (int i = 0; i < 3; ++i) f.q[i] = FGwf{(gcq::gwf::Ring*)q[i % 2]};
+        return fn(f);
+      }
+      case GCQ_GLFQ: {
+        Frontiers<FLfq> f;
+        for (int i = 0; i < 3; ++i) f.q[i] = FLfq{(gcq::lfq::Ring*)q[i % 2]};
+        return fn(f);
+      }
+      case GCQ_SFQ: {
+        Frontiers<FSfq> f;
+        for (int i = 0; i < 3; ++i) f.q[i] = FSfq{(gcq::sfq::Queue*)q[i % 2]};
+        return fn(f);
+      }
+      case GCQ_WFQ: {
+        Frontiers<FYmc> f;
+        for (int i = 0; i < 3; ++i) f.q[i] = FYmc{(gcq::ymc::Queue*)q[i], (gcq::ymc::Handle*)h[i]};
+        return fn(f);
+      }
+    }
+    GCQ_REQUIRE(false, "frontier set not allocated");
+  }
+};
+
+}  // namespace gcqf

@@ -12,6 +12,26 @@ class BfsResult(C.Structure):
                 ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("threads", C.c_uint32), ("block", C.c_uint32), ("pop_width", C.c_uint32),
+                ("flush_at", C.c_uint32), ("capacity_log2", C.c_uint32), ("use_graph", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("reached", C.c_uint64), ("edges_scanned", C.c_uint64), ("levels", C.c_int32),
+                ("status", C.c_int32)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double), ("edges_scanned", C.c_uint64),
+                ("levels", C.c_int32), ("graph_replayed", C.c_int32)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("cfg", BatchConfig), ("ms", C.c_double)]
+
+
 _P = C.POINTER
 VP = C.c_void_p
 
@@ -24,6 +44,12 @@ SIGNATURES = {
     "gcq_graph_pick_source": (C.c_int32, [VP, C.c_uint64, C.c_uint64]),
     "gcq_graph_destroy": (C.c_int, [VP]),
     "gcq_bfs_run": (C.c_int, [VP, C.c_int32, _P(BfsConfig), VP, _P(BfsResult)]),
+    "gcq_batch_create": (C.c_int, [_P(VP), C.c_int32, _P(VP)]),
+    "gcq_batch_config_default": (None, [_P(BatchConfig)]),
+    "gcq_batch_run": (C.c_int, [VP, _P(BatchConfig), C.c_int32, VP, VP, _P(VP), _P(QueryResult), _P(BatchResult)]),
+    "gcq_batch_autotune": (C.c_int, [VP, C.c_int32, C.c_int32, VP, VP, C.c_int32, _P(BatchConfig), _P(TuneTrial),
+                                     C.c_int32, _P(C.c_int32)]),
+    "gcq_batch_destroy": (C.c_int, [VP]),
 }
 
 

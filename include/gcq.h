@@ -204,6 +204,57 @@ typedef struct gcq_bfs_result {
  * bfs_expand_kernel + bfs_hub_kernel (bfs_bench.cpp:327-633). */
 int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg, int32_t* depth_out, gcq_bfs_result* res);
 
+/* ------------------------------------------------------------------ batched BFS
+ * No counterpart in the reference (its BFS harness runs one query per
+ * process, bfs_bench.cpp:653-800).  Many traversals, over any of a set of
+ * resident graphs, advance together in one persistent kernel that pulls
+ * (query slot, vertex) items through one shared frontier queue; each
+ * query's depth array is bit-exact with a plain run. */
+typedef struct gcq_batch gcq_batch;
+typedef struct gcq_batch_config {
+  int32_t kind;           /* frontier queue: enum gcq_kind */
+  uint32_t threads;       /* persistent grid, multiple of block; all blocks co-resident */
+  uint32_t block;         /* 64..1024, multiple of 64 */
+  uint32_t pop_width;     /* aggregation width: lanes of a wave that dequeue per step (8..64) */
+  uint32_t flush_at;      /* staged discoveries per wave before an aggregated enqueue (8..64) */
+  uint32_t capacity_log2; /* frontier ring size; 0 = smallest power of two >= sum of query V */
+  uint32_t use_graph;     /* capture the launch sequence in a hipGraph and replay it */
+  uint32_t pad;
+} gcq_batch_config;
+typedef struct gcq_query_result {
+  uint64_t reached;
+  uint64_t edges_scanned;
+  int32_t levels;
+  int32_t status;         /* 0 = done */
+} gcq_query_result;
+typedef struct gcq_batch_result {
+  double total_ms;        /* host wall time of the whole batch, resets and statistics included */
+  double kernel_ms;       /* device time of the traversal kernel */
+  uint64_t edges_scanned; /* over all queries */
+  int32_t levels;         /* global levels the batch ran */
+  int32_t graph_replayed; /* 1 if the launch was a replay of a captured graph */
+} gcq_batch_result;
+/* A batch context over `ngraphs` graphs (borrowed: they must outlive it). */
+int gcq_batch_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_batch** out);
+void gcq_batch_config_default(gcq_batch_config* cfg);
+/* Run nq queries (graph_ids[i], sources[i]) as one batch.  depth_out, if
+ * not NULL, holds nq host pointers (each NULL or room for that graph's nv
+ * int32 depths); per_query, if not NULL, receives nq results. */
+int gcq_batch_run(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32_t* graph_ids,
+                  const int32_t* sources, int32_t* const* depth_out, gcq_query_result* per_query,
+                  gcq_batch_result* res);
+/* Project-written autotuner: times the batch under candidate configurations
+ * (block, threads, pop_width, flush_at, capacity) and writes the fastest to
+ * best; trials_out, if not NULL, receives up to max_trials (config, ms) rows. */
+typedef struct gcq_tune_trial {
+  gcq_batch_config cfg;
+  double ms;
+} gcq_tune_trial;
+int gcq_batch_autotune(gcq_batch* b, int32_t kind, int32_t nq, const int32_t* graph_ids, const int32_t* sources,
+                       int32_t reps, gcq_batch_config* best, gcq_tune_trial* trials_out, int32_t max_trials,
+                       int32_t* ntrials);
+int gcq_batch_destroy(gcq_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
