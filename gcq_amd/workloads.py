@@ -32,6 +32,23 @@ class TuneTrial(C.Structure):
     _fields_ = [("cfg", BatchConfig), ("ms", C.c_double)]
 
 
+class RtSceneDesc(C.Structure):
+    _fields_ = [("spheres", C.c_void_p), ("nspheres", C.c_int32), ("nplanes", C.c_int32), ("planes", C.c_void_p),
+                ("cam_o", C.c_float * 3), ("cam_f", C.c_float * 3), ("cam_r", C.c_float * 3), ("cam_u", C.c_float * 3),
+                ("light", C.c_float * 3), ("ambient", C.c_float * 3)]
+
+
+class RtConfig(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("threads_per_tile", C.c_uint32), ("block", C.c_uint32), ("max_depth", C.c_int32),
+                ("width", C.c_int32), ("height", C.c_int32), ("tile_w", C.c_int32), ("tile_h", C.c_int32),
+                ("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32), ("h", C.c_int32)]
+
+
+class RtResult(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("total_rays", C.c_uint64), ("kernel_ms", C.c_double),
+                ("total_ms", C.c_double), ("tiles", C.c_int32), ("pad", C.c_int32)]
+
+
 _P = C.POINTER
 VP = C.c_void_p
 
@@ -50,6 +67,12 @@ SIGNATURES = {
     "gcq_batch_autotune": (C.c_int, [VP, C.c_int32, C.c_int32, VP, VP, C.c_int32, _P(BatchConfig), _P(TuneTrial),
                                      C.c_int32, _P(C.c_int32)]),
     "gcq_batch_destroy": (C.c_int, [VP]),
+    "gcq_rt_scene_create": (C.c_int, [_P(RtSceneDesc), _P(VP)]),
+    "gcq_rt_scene_preset": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, _P(VP)]),
+    "gcq_rt_scene_get": (C.c_int, [VP, _P(RtSceneDesc)]),
+    "gcq_rt_scene_destroy": (C.c_int, [VP]),
+    "gcq_rt_config_default": (None, [_P(RtConfig)]),
+    "gcq_rt_render": (C.c_int, [VP, _P(RtConfig), VP, _P(RtResult)]),
 }
 
 

@@ -255,6 +255,59 @@ int gcq_batch_autotune(gcq_batch* b, int32_t kind, int32_t nq, const int32_t* gr
                        int32_t* ntrials);
 int gcq_batch_destroy(gcq_batch* b);
 
+/* ------------------------------------------------------------------ ray tracing
+ * Queue-compacted Whitted tracer of rt_final.cpp (types :343-362, shading
+ * :517-645, scenes :650-684, camera :849-859): spheres and planes, one
+ * directional light, mirror bounces up to max_depth, sky on a miss.  Each
+ * tile has one queue holding the local pixel indices of its live bounce
+ * rays; a pixel's contributions are summed in bounce order, so the image is
+ * independent of scheduling.  Replaces k_generate + k_trace + main. */
+typedef struct gcq_rt_sphere { float c[3]; float r; float albedo[3]; float refl; } gcq_rt_sphere;  /* rt_final.cpp:343 */
+typedef struct gcq_rt_plane { float n[3]; float d; float albedo[3]; float refl; } gcq_rt_plane;    /* rt_final.cpp:344 */
+typedef struct gcq_rt_scene_desc {
+  const gcq_rt_sphere* spheres;
+  int32_t nspheres;
+  int32_t nplanes;
+  const gcq_rt_plane* planes;
+  float cam_o[3];
+  float cam_f[3], cam_r[3], cam_u[3]; /* forward/right/up as given; normalised on the host (rt_final.cpp:856-859) */
+  float light[3];                     /* direction towards the light, normalised in the kernel (:560) */
+  float ambient[3];                   /* (:561) */
+} gcq_rt_scene_desc;
+typedef struct gcq_rt_scene gcq_rt_scene;
+enum gcq_rt_preset {
+  GCQ_RT_COMPLEX = 0,  /* build_scene_complex + its camera (rt_final.cpp:650-671, :850-852) */
+  GCQ_RT_CORNELL = 1,  /* build_scene_cornell + its camera (:673-684, :853-855) */
+  GCQ_RT_RANDOM = 2    /* n random spheres on a ground plane (no counterpart in the reference) */
+};
+int gcq_rt_scene_create(const gcq_rt_scene_desc* desc, gcq_rt_scene** out);
+/* nspheres and seed are used by GCQ_RT_RANDOM only. */
+int gcq_rt_scene_preset(int32_t preset, int32_t nspheres, uint64_t seed, gcq_rt_scene** out);
+/* Fills desc with pointers into the scene's host copy (valid until destroy). */
+int gcq_rt_scene_get(const gcq_rt_scene* s, gcq_rt_scene_desc* desc);
+int gcq_rt_scene_destroy(gcq_rt_scene* s);
+
+typedef struct gcq_rt_config {
+  int32_t kind;              /* per-tile queue: enum gcq_kind */
+  uint32_t threads_per_tile; /* workers per tile, multiple of 64 (ref argv[3], default 1024) */
+  uint32_t block;            /* (ref TPB 256) */
+  int32_t max_depth;         /* bounces (ref argv[2], default 4) */
+  int32_t width, height;     /* full image (camera aspect and pixel centres) */
+  int32_t tile_w, tile_h;    /* tiling of the rendered region (ref 160 x 90) */
+  int32_t x0, y0, w, h;      /* rendered region; w = h = 0 means the whole image */
+} gcq_rt_config;
+typedef struct gcq_rt_result {
+  uint64_t primary_rays;
+  uint64_t total_rays;       /* rays traced, bounces included (ref traced) */
+  double kernel_ms;          /* the one trace launch */
+  double total_ms;           /* host wall time, resets and read-back included */
+  int32_t tiles;
+  int32_t pad;
+} gcq_rt_result;
+void gcq_rt_config_default(gcq_rt_config* cfg);
+/* Render cfg's region into img_out (w * h RGB float triples, row-major). */
+int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* img_out, gcq_rt_result* res);
+
 #ifdef __cplusplus
 }
 #endif

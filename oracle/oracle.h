@@ -61,6 +61,14 @@ long orc_bfs_serial(int32_t nv, const int32_t* offs, const int32_t* idx, int32_t
 long orc_bfs_parallel(int32_t nv, const int32_t* offs, const int32_t* idx, int32_t src, int32_t* depth,
                       int threads);
 
+/* Reference ray tracer image of region (x0, y0, w, h) of a W x H frame:
+ * spheres / planes as 8 floats each (centre|normal, radius|d, albedo, refl),
+ * cam12 = origin, forward, right, up (not yet normalised), light direction
+ * (not yet normalised), ambient.  img gets w*h RGB floats. */
+int orc_rt_render(const float* sph, int32_t ns, const float* pl, int32_t np, const float* cam12, const float* light3,
+                  const float* ambient3, int32_t W, int32_t H, int32_t x0, int32_t y0, int32_t w, int32_t h,
+                  int32_t max_depth, int32_t threads, float* img, uint64_t* rays_traced);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,10 @@ def load():
                                        C.POINTER(C.c_uint64)]
         lib.orc_bfs_parallel.restype = C.c_long
         lib.orc_bfs_parallel.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int]
+        lib.orc_rt_render.restype = C.c_int
+        lib.orc_rt_render.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_void_p, C.c_void_p]
         _lib = lib
     return _lib
 
@@ -128,3 +132,21 @@ def bfs_parallel(offsets, indices, src, threads=0):
     idx = np.ascontiguousarray(indices, dtype=np.int32)
     reached = load().orc_bfs_parallel(nv, off.ctypes.data, idx.ctypes.data, src, depth.ctypes.data, threads)
     return depth, reached
+
+
+def rt_render(spheres, planes, cam12, light3, ambient3, W, H, x0=0, y0=0, w=0, h=0, max_depth=4, threads=0):
+    """CPU restatement of the reference ray tracer; returns (image[h, w, 3], rays traced)."""
+    w = w or W
+    h = h or H
+    sph = np.ascontiguousarray(spheres, np.float32).reshape(-1, 8)
+    pl = np.ascontiguousarray(planes, np.float32).reshape(-1, 8)
+    cam = np.ascontiguousarray(cam12, np.float32)
+    li = np.ascontiguousarray(light3, np.float32)
+    am = np.ascontiguousarray(ambient3, np.float32)
+    img = np.empty((h, w, 3), np.float32)
+    rays = C.c_uint64(0)
+    rc = load().orc_rt_render(sph.ctypes.data, len(sph), pl.ctypes.data, len(pl), cam.ctypes.data, li.ctypes.data,
+                              am.ctypes.data, W, H, x0, y0, w, h, max_depth, threads, img.ctypes.data, C.byref(rays))
+    if rc != 0:
+        raise ValueError("orc_rt_render: bad region")
+    return img, rays.value
