@@ -26,20 +26,10 @@
 #include <type_traits>
 #include <vector>
 
+#include "bfs_batch.hpp"
 #include "graph_handle.hpp"
 
 namespace bk {
-
-using namespace gcqf;
-
-// One query, read by the kernel with device-scope loads.
-struct Slot {
-  uint64_t off, idx, depth;  // CSR of its graph, its depth array
-  uint32_t start, src;       // level at which it starts; its source
-  uint32_t nv, pad;
-  uint64_t pad2[3];
-};
-static_assert(sizeof(Slot) == 64, "slot");
 
 struct Ctl {
   Barrier bar;
@@ -48,94 +38,8 @@ struct Ctl {
   int32_t levels, p2[31];
 };
 
-constexpr uint32_t kAliveStride = 32;  // one 128-B line per query
-
-struct Dev {
-  const Slot* slots;
-  uint32_t* alive;  // [slot * kAliveStride]: last level at which the query has work
-  int32_t* done;    // per slot: levels once finished, -1 while running
-  Ctl* c;
-  uint32_t nslots, vbits, pop_width, flush_at;
-};
-
-__device__ __forceinline__ uint64_t shfl64(uint64_t x, int lane) {
-  const uint32_t lo = __shfl((uint32_t)x, lane, 64);
-  const uint32_t hi = __shfl((uint32_t)(x >> 32), lane, 64);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Drain the level's input queue; returns the lane's discoveries.  All 64
-// lanes of the wave call it together.
 template <class F>
-__device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L, uint32_t tid, uint32_t* stage) {
-  const uint32_t lane = lane_id();
-  const uint32_t vmask = (1u << d.vbits) - 1u;
-  uint32_t sc = 0, disc_n = 0;
-  for (;;) {
-    uint32_t item = 0;
-    bool got = false;
-    if (lane < d.pop_width) got = qin.pop(tid, &item);
-    if (__ballot(got) == 0) break;  // every popping lane saw the level's queue empty
-    uint32_t slot = 0;
-    int32_t beg = 0, deg = 0, dn = 0;
-    uint64_t idxp = 0, depp = 0;
-    if (got) {
-      slot = item >> d.vbits;
-      const uint32_t v = item & vmask;
-      const Slot* s = &d.slots[slot];
-      const int32_t* off = (const int32_t*)gcq::ld_rlx(&s->off);
-      idxp = gcq::ld_rlx(&s->idx);
-      depp = gcq::ld_rlx(&s->depth);
-      dn = L - (int32_t)gcq::ld_rlx(&s->start) + 1;
-      beg = off[v];  // CSR arrays are uploaded before launch and never written
-      deg = off[v + 1] - beg;
-    }
-    int32_t incl = deg;  // inclusive prefix of degrees across the wave
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t t = __shfl_up(incl, o, 64);
-      if ((int)lane >= o) incl += t;
-    }
-    const int32_t excl = incl - deg;
-    const int32_t total = __shfl(incl, 63, 64);
-    for (int32_t base = 0; base < total; base += 64) {
-      const int32_t e = base + (int32_t)lane;
-      const int32_t ec = e < total ? e : total - 1;
-      int lo = 0, hi = 63;  // owner = first lane whose inclusive prefix exceeds ec
-      for (int s = 0; s < 6; ++s) {
-        const int mid = (lo + hi) >> 1;
-        if (__shfl(incl, mid, 64) > ec) hi = mid;
-        else lo = mid + 1;
-      }
-      const int32_t obeg = __shfl(beg, lo, 64);
-      const int32_t oexcl = __shfl(excl, lo, 64);
-      const uint32_t oslot = __shfl(slot, lo, 64);
-      const int32_t odn = __shfl(dn, lo, 64);
-      const uint64_t oidx = shfl64(idxp, lo);
-      const uint64_t odep = shfl64(depp, lo);
-      bool disc = false;
-      uint32_t w = 0;
-      if (e < total) {
-        w = (uint32_t)((const int32_t*)oidx)[obeg + (ec - oexcl)];
-        int32_t* dp = (int32_t*)odep + w;
-        if (gcq::ld_rlx(dp) == -1) {
-          int32_t expect = -1;
-          disc = gcq::cas_rlx(dp, expect, odn);
-        }
-      }
-      if (disc) gcq::st_rlx(&d.alive[oslot * kAliveStride], (uint32_t)(L + 1));
-      const uint64_t dm = __ballot(disc);
-      if (disc) stage[sc + gcq::rank_in(dm)] = (oslot << d.vbits) | w;
-      sc += (uint32_t)__builtin_popcountll(dm);
-      disc_n += disc ? 1u : 0u;
-      while (sc >= d.flush_at) flush(qout, tid, stage, sc, sc < 64u ? sc : 64u, &d.c->err);
-    }
-  }
-  if (sc) flush(qout, tid, stage, sc, sc, &d.c->err);
-  return disc_n;
-}
-
-template <class F>
-__global__ void k_batch(Frontiers<F> fq, Dev d) {
+__global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
@@ -144,14 +48,14 @@ __global__ void k_batch(Frontiers<F> fq, Dev d) {
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
     const uint32_t dw = wave_sum32(expand(fq.in(L), fq.out(L), d, L, tid, st));
-    if (lane_id() == 0 && dw) gcq::faa_rlx(&d.c->next[L % 3], dw);
-    if (!grid_barrier(&d.c->bar, gridDim.x, [&]() {
+    if (lane_id() == 0 && dw) gcq::faa_rlx(&ctl->next[L % 3], dw);
+    if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
         }))
       break;
-    const uint32_t n = gcq::ld_rlx(&d.c->next[L % 3]);
+    const uint32_t n = gcq::ld_rlx(&ctl->next[L % 3]);
     if (blockIdx.x == 0) {
-      if (threadIdx.x == 0) gcq::st_rlx(&d.c->next[(L + 2) % 3], 0u);
+      if (threadIdx.x == 0) gcq::st_rlx(&ctl->next[(L + 2) % 3], 0u);
       // Queries with nothing discovered at level L are finished.  alive only
       // grows, so a block already at level L+1 cannot hide a finish.
       for (uint32_t s = threadIdx.x; s < d.nslots; s += blockDim.x) {
@@ -162,7 +66,7 @@ __global__ void k_batch(Frontiers<F> fq, Dev d) {
     }
     if (n == 0) break;
   }
-  if (tid == 0) d.c->levels = L + 1;
+  if (tid == 0) ctl->levels = L + 1;
 }
 
 // Level-0 frontier: one item per query.  The grid has at most as many
@@ -173,7 +77,7 @@ __global__ void k_seed(F q, Dev d) {
   for (uint32_t s = tid; s < d.nslots; s += gridDim.x * blockDim.x) {
     const Slot& sl = d.slots[s];
     ((int32_t*)sl.depth)[sl.src] = 0;
-    if (!q.push(tid, (s << d.vbits) | sl.src)) gcq::or_rlx(&d.c->err, 1u);
+    if (!q.push(tid, (s << d.vbits) | sl.src)) gcq::or_rlx(d.err, 1u);
   }
 }
 
@@ -297,7 +201,7 @@ int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq, const bk::Dev& d, 
   b->fs.reset(s);
   const uint32_t seed_blocks = std::min((nq + 63) / 64, cfg->threads / 64);
   hipLaunchKernelGGL(bk::k_seed<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], d);
-  hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d);
+  hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d, b->ctl);
   const uint32_t bx = std::max<uint32_t>(1, std::min<uint32_t>(64, (max_nv + 255) / 256));
   hipLaunchKernelGGL(bk::k_stats, dim3(bx, nq), dim3(256), 0, s, d, b->stats);
   GCQ_HIP(hipGetLastError());
@@ -357,7 +261,7 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
     b->h_slots[i] = s;
   }
   GCQ_HIP(hipMemcpyAsync(b->d_slots, b->h_slots, sizeof(bk::Slot) * nq, hipMemcpyHostToDevice, b->stream));
-  bk::Dev d{b->d_slots, b->alive, b->done, b->ctl, (uint32_t)nq, b->vbits, cfg->pop_width, cfg->flush_at};
+  bk::Dev d{b->d_slots, b->alive, b->done, &b->ctl->err, (uint32_t)nq, b->vbits, cfg->pop_width, cfg->flush_at};
 
   // Every block of the persistent kernel must be resident.
   int dev = 0, cus = 0, per_cu = 0;

@@ -1,0 +1,111 @@
+// bfs_batch.hpp — the multi-query BFS level expansion shared by the batched
+// (batch.hip) and served (serve.hip) persistent kernels.
+//
+// Work items are (query slot, vertex) packed in 32 bits: the vertex in the
+// low `vbits` bits, the slot above it, below 2^31 so no queue sentinel can
+// occur.  A vertex discovered while expanding level L of a query that
+// started at level `start` gets depth L - start + 1 through a CAS on -1
+// (bfs_bench.cpp:383-406), so each query's depths are exactly BFS depths.
+#pragma once
+#include "frontier.hpp"
+
+namespace bk {
+
+using namespace gcqf;
+
+// One query, read by the kernel with device-scope loads.
+struct Slot {
+  uint64_t off, idx, depth;  // CSR of its graph, its depth array
+  uint32_t start, src;       // level at which it starts; its source
+  uint32_t nv, pad;
+  uint64_t pad2[3];
+};
+static_assert(sizeof(Slot) == 64, "slot");
+
+constexpr uint32_t kAliveStride = 32;  // one 128-B line per query
+
+struct Dev {
+  const Slot* slots;
+  uint32_t* alive;  // [slot * kAliveStride]: last level at which the query has work
+  int32_t* done;    // per slot: levels once finished, -1 while running
+  uint32_t* err;    // bit 0: a frontier enqueue failed
+  uint32_t nslots, vbits, pop_width, flush_at;
+};
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int lane) {
+  const uint32_t lo = __shfl((uint32_t)x, lane, 64);
+  const uint32_t hi = __shfl((uint32_t)(x >> 32), lane, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Drain the level's input queue; returns the lane's discoveries.  All 64
+// lanes of the wave call it together.
+template <class F>
+__device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L, uint32_t tid, uint32_t* stage) {
+  const uint32_t lane = lane_id();
+  const uint32_t vmask = (1u << d.vbits) - 1u;
+  uint32_t sc = 0, disc_n = 0;
+  for (;;) {
+    uint32_t item = 0;
+    bool got = false;
+    if (lane < d.pop_width) got = qin.pop(tid, &item);
+    if (__ballot(got) == 0) break;  // every popping lane saw the level's queue empty
+    uint32_t slot = 0;
+    int32_t beg = 0, deg = 0, dn = 0;
+    uint64_t idxp = 0, depp = 0;
+    if (got) {
+      slot = item >> d.vbits;
+      const uint32_t v = item & vmask;
+      const Slot* s = &d.slots[slot];
+      const int32_t* off = (const int32_t*)gcq::ld_rlx(&s->off);
+      idxp = gcq::ld_rlx(&s->idx);
+      depp = gcq::ld_rlx(&s->depth);
+      dn = L - (int32_t)gcq::ld_rlx(&s->start) + 1;
+      beg = off[v];  // CSR arrays are uploaded before launch and never written
+      deg = off[v + 1] - beg;
+    }
+    int32_t incl = deg;  // inclusive prefix of degrees across the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t t = __shfl_up(incl, o, 64);
+      if ((int)lane >= o) incl += t;
+    }
+    const int32_t excl = incl - deg;
+    const int32_t total = __shfl(incl, 63, 64);
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t e = base + (int32_t)lane;
+      const int32_t ec = e < total ? e : total - 1;
+      int lo = 0, hi = 63;  // owner = first lane whose inclusive prefix exceeds ec
+      for (int s = 0; s < 6; ++s) {
+        const int mid = (lo + hi) >> 1;
+        if (__shfl(incl, mid, 64) > ec) hi = mid;
+        else lo = mid + 1;
+      }
+      const int32_t obeg = __shfl(beg, lo, 64);
+      const int32_t oexcl = __shfl(excl, lo, 64);
+      const uint32_t oslot = __shfl(slot, lo, 64);
+      const int32_t odn = __shfl(dn, lo, 64);
+      const uint64_t oidx = shfl64(idxp, lo);
+      const uint64_t odep = shfl64(depp, lo);
+      bool disc = false;
+      uint32_t w = 0;
+      if (e < total) {
+        w = (uint32_t)((const int32_t*)oidx)[obeg + (ec - oexcl)];
+        int32_t* dp = (int32_t*)odep + w;
+        if (gcq::ld_rlx(dp) == -1) {
+          int32_t expect = -1;
+          disc = gcq::cas_rlx(dp, expect, odn);
+        }
+      }
+      if (disc) gcq::st_rlx(&d.alive[oslot * kAliveStride], (uint32_t)(L + 1));
+      const uint64_t dm = __ballot(disc);
+      if (disc) stage[sc + gcq::rank_in(dm)] = (oslot << d.vbits) | w;
+      sc += (uint32_t)__builtin_popcountll(dm);
+      disc_n += disc ? 1u : 0u;
+      while (sc >= d.flush_at) flush(qout, tid, stage, sc, sc < 64u ? sc : 64u, d.err);
+    }
+  }
+  if (sc) flush(qout, tid, stage, sc, sc, d.err);
+  return disc_n;
+}
+
+}  // namespace bk

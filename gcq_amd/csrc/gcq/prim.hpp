@@ -32,6 +32,15 @@ template <class T>
 __device__ __forceinline__ T ld_rlx(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// System scope: words in host-pinned memory shared with host threads.
+template <class T>
+__device__ __forceinline__ T ld_sys(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T>
+__device__ __forceinline__ void st_sys(T* p, nd_t<T> v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 template <class T>
 __device__ __forceinline__ T ld_acq(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);

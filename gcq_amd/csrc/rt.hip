@@ -34,84 +34,9 @@
 #include <type_traits>
 #include <vector>
 
-#include "frontier.hpp"
+#include "rt_handle.hpp"
 
 namespace rtk {
-
-struct V3 {
-  float x, y, z;
-};
-__host__ __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
-__host__ __device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__host__ __device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__host__ __device__ __forceinline__ V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-__host__ __device__ __forceinline__ V3 operator*(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
-__host__ __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__host__ __device__ __forceinline__ V3 norm(V3 v) {
-  const float d = sqrtf(dot(v, v));
-  return d > 0.f ? v * (1.f / d) : v;
-}
-__host__ __device__ __forceinline__ V3 reflect(V3 v, V3 n) { return v - n * (2.f * dot(v, n)); }
-__host__ __device__ __forceinline__ V3 clamp01(V3 c) {
-  return v3(fminf(fmaxf(c.x, 0.f), 1.f), fminf(fmaxf(c.y, 0.f), 1.f), fminf(fmaxf(c.z, 0.f), 1.f));
-}
-__host__ __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
-
-// A live bounce ray; also holds its pixel's running sum.
-struct alignas(16) RayRec {
-  float o[3];
-  int32_t depth;
-  float d[3];
-  float pad0;
-  float at[3];
-  float pad1;
-  float acc[3];
-  float pad2;
-};
-static_assert(sizeof(RayRec) == 64, "ray record");
-
-struct Ray {
-  V3 o, d, at, acc;
-  int32_t depth;
-};
-
-__device__ __forceinline__ void store_ray(RayRec* r, const Ray& x) {
-  uint64_t* w = (uint64_t*)r;
-  auto pk = [](float a, float b) {
-    return (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
-  };
-  gcq::st_rlx(&w[0], pk(x.o.x, x.o.y));
-  gcq::st_rlx(&w[1], pk(x.o.z, __int_as_float(x.depth)));
-  gcq::st_rlx(&w[2], pk(x.d.x, x.d.y));
-  gcq::st_rlx(&w[3], pk(x.d.z, 0.f));
-  gcq::st_rlx(&w[4], pk(x.at.x, x.at.y));
-  gcq::st_rlx(&w[5], pk(x.at.z, 0.f));
-  gcq::st_rlx(&w[6], pk(x.acc.x, x.acc.y));
-  gcq::st_rlx(&w[7], pk(x.acc.z, 0.f));
-}
-__device__ __forceinline__ Ray load_ray(const RayRec* r) {
-  const uint64_t* w = (const uint64_t*)r;
-  uint64_t a[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) a[i] = gcq::ld_rlx(&w[i]);
-  auto lo = [](uint64_t x) { return __uint_as_float((uint32_t)x); };
-  auto hi = [](uint64_t x) { return __uint_as_float((uint32_t)(x >> 32)); };
-  Ray x;
-  x.o = v3(lo(a[0]), hi(a[0]), lo(a[1]));
-  x.depth = __float_as_int(hi(a[1]));
-  x.d = v3(lo(a[2]), hi(a[2]), lo(a[3]));
-  x.at = v3(lo(a[4]), hi(a[4]), lo(a[5]));
-  x.acc = v3(lo(a[6]), hi(a[6]), lo(a[7]));
-  return x;
-}
-
-struct Scene {
-  const gcq_rt_sphere* sph;
-  const gcq_rt_plane* pl;
-  int32_t ns, np;
-  V3 cam_o, cam_f, cam_r, cam_u;  // f, r, u already normalised on the host
-  V3 light_raw, ambient;
-};
 
 struct Job {
   int32_t W, H, x0, y0, w, h, tile_w, tile_h, tiles_x, max_depth;
@@ -124,91 +49,6 @@ struct Job {
 };
 constexpr uint32_t kCounterStride = 16;
 constexpr uint32_t kIdleLimit = 1u << 24;
-
-// Nearest hit (trace_scene, rt_final.cpp:403-419): spheres in order, then
-// planes; a later object replaces the best only if strictly nearer.
-struct Hit {
-  float t;
-  int32_t kind, idx;  // kind 0 none, 1 sphere, 2 plane
-};
-__device__ __forceinline__ Hit nearest(const Scene& sc, V3 o, V3 d) {
-  Hit h{1e30f, 0, -1};
-  for (int32_t i = 0; i < sc.ns; ++i) {
-    const gcq_rt_sphere& s = sc.sph[i];
-    const V3 oc = o - ld3(s.c);
-    const float b = dot(oc, d);
-    const float c = dot(oc, oc) - s.r * s.r;
-    const float disc = b * b - c;
-    if (disc >= 0.f) {
-      const float t = -b - sqrtf(disc);
-      if (t > 1e-4f && t < h.t) h = Hit{t, 1, i};
-    }
-  }
-  for (int32_t i = 0; i < sc.np; ++i) {
-    const gcq_rt_plane& p = sc.pl[i];
-    const V3 n = ld3(p.n);
-    const float denom = dot(n, d);
-    if (fabsf(denom) > 1e-5f) {
-      const float t = -(dot(n, o) + p.d) / denom;
-      if (t > 1e-4f && t < h.t) h = Hit{t, 2, i};
-    }
-  }
-  return h;
-}
-
-// Primary ray of pixel (px, py) (k_generate, rt_final.cpp:482-495).
-__device__ __forceinline__ Ray primary(const Scene& sc, const Job& jb, int32_t px, int32_t py) {
-  const float aspect = (float)jb.W / (float)jb.H;
-  const float u = ((px + 0.5f) / jb.W - 0.5f) * 2.f * aspect;
-  const float v = -((py + 0.5f) / jb.H - 0.5f) * 2.f;
-  Ray r;
-  r.o = sc.cam_o;
-  r.d = norm(sc.cam_f + sc.cam_r * u + sc.cam_u * v);
-  r.at = v3(1.f, 1.f, 1.f);
-  r.acc = v3(0.f, 0.f, 0.f);
-  r.depth = 0;
-  return r;
-}
-
-// One bounce of ray r (k_trace body, rt_final.cpp:572-628).  Returns true
-// and updates r in place if the chain continues.
-__device__ __forceinline__ bool shade(const Scene& sc, const Job& jb, V3 light, Ray& r) {
-  const Hit h = nearest(sc, r.o, r.d);
-  if (h.kind == 0) {
-    const float t = 0.5f * (r.d.y + 1.f);
-    const V3 sky = v3(1.f, 1.f, 1.f) * (1.f - t) + v3(0.5f, 0.7f, 1.f) * t;
-    r.acc = r.acc + clamp01(sky * r.at);
-    return false;
-  }
-  const V3 p = r.o + r.d * h.t;
-  V3 n, albedo;
-  float refl;
-  if (h.kind == 1) {
-    const gcq_rt_sphere& s = sc.sph[h.idx];
-    n = norm(p - ld3(s.c));
-    albedo = ld3(s.albedo);
-    refl = s.refl;
-  } else {
-    const gcq_rt_plane& q = sc.pl[h.idx];
-    n = ld3(q.n);
-    albedo = ld3(q.albedo);
-    refl = q.refl;
-  }
-  const float ndl = fmaxf(0.f, dot(n, light));
-  const V3 direct = sc.ambient + albedo * ndl;
-  const V3 contrib = direct * r.at;
-  if (refl > 0.f && r.depth < jb.max_depth) {
-    const V3 nr = clamp01(contrib * (1.f - refl));
-    if (nr.x > 0.f || nr.y > 0.f || nr.z > 0.f) r.acc = r.acc + nr;
-    r.o = p + n * 1e-4f;
-    r.d = norm(reflect(r.d, n));
-    r.depth = r.depth + 1;
-    r.at = r.at * albedo * refl;
-    return true;
-  }
-  r.acc = r.acc + clamp01(contrib);
-  return false;
-}
 
 // All tiles of a region in one launch: blockIdx.y is the tile, the
 // threads_per_tile threads of blockIdx.x are its workers.
@@ -234,7 +74,7 @@ __global__ void k_render(const F* __restrict__ queues, Scene sc, Job jb) {
     if (next_primary < npx) {
       i = next_primary;
       next_primary += T;
-      r = primary(sc, jb, jb.x0 + lx0 + (int32_t)(i % (uint32_t)tw), jb.y0 + ly0 + (int32_t)(i / (uint32_t)tw));
+      r = primary(sc, jb.W, jb.H, jb.x0 + lx0 + (int32_t)(i % (uint32_t)tw), jb.y0 + ly0 + (int32_t)(i / (uint32_t)tw));
       have = true;
     } else if (q.pop(wt, &i)) {
       r = load_ray(&jb.recs[(ly0 + (int32_t)(i / (uint32_t)tw)) * jb.w + lx0 + (int32_t)(i % (uint32_t)tw)]);
@@ -259,7 +99,7 @@ __global__ void k_render(const F* __restrict__ queues, Scene sc, Job jb) {
     if (have) {
       ++traced;
       const uint32_t rp = (uint32_t)((ly0 + (int32_t)(i / (uint32_t)tw)) * jb.w + lx0 + (int32_t)(i % (uint32_t)tw));
-      if (shade(sc, jb, light, r)) {
+      if (shade(sc, jb.max_depth, light, r)) {
         store_ray(&jb.recs[rp], r);
         gcq::vm_wait();  // record complete before its index is published
         if (!q.push(wt, i)) {
@@ -312,33 +152,7 @@ __global__ void k_init_tiles_ymc(char* pool, uint64_t stride, char* hpool, uint6
 
 }  // namespace rtk
 
-struct gcq_rt_scene {
-  std::vector<gcq_rt_sphere> sph;
-  std::vector<gcq_rt_plane> pl;
-  gcq_rt_scene_desc desc;  // raw camera / light, pointers into sph / pl
-  // device side (lazy)
-  gcq_rt_sphere* d_sph = nullptr;
-  gcq_rt_plane* d_pl = nullptr;
-  hipStream_t stream = nullptr;
-  // render context
-  int kind = -1;
-  uint32_t ntiles = 0, cap = 0, threads = 0;
-  uint64_t qstride = 0, hstride = 0, ymc_cells = 0;
-  char* qpool = nullptr;
-  char* hpool = nullptr;
-  void* adapters = nullptr;
-  rtk::RayRec* recs = nullptr;
-  uint64_t recs_cap = 0;
-  float* img = nullptr;
-  uint64_t img_cap = 0;
-  uint64_t* counters = nullptr;  // alive, traced per tile
-  uint32_t counters_cap = 0;
-  uint32_t* err = nullptr;
-};
-
 namespace {
-
-rtk::V3 host_norm(const float* v) { return rtk::norm(rtk::v3(v[0], v[1], v[2])); }
 
 void fix_desc(gcq_rt_scene* s) {
   s->desc.spheres = s->sph.data();
@@ -361,23 +175,6 @@ void free_ctx(gcq_rt_scene* s) {
   s->counters_cap = 0;
   s->kind = -1;
   s->ntiles = s->cap = s->threads = 0;
-}
-
-int ensure_device(gcq_rt_scene* s) {
-  if (s->stream) return GCQ_OK;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-    gcqh::set_error("no HIP device visible");
-    return GCQ_ENODEV;
-  }
-  GCQ_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  GCQ_HIP(hipMalloc(&s->d_sph, sizeof(gcq_rt_sphere) * std::max<size_t>(1, s->sph.size())));
-  GCQ_HIP(hipMalloc(&s->d_pl, sizeof(gcq_rt_plane) * std::max<size_t>(1, s->pl.size())));
-  if (!s->sph.empty())
-    GCQ_HIP(hipMemcpy(s->d_sph, s->sph.data(), sizeof(gcq_rt_sphere) * s->sph.size(), hipMemcpyHostToDevice));
-  if (!s->pl.empty())
-    GCQ_HIP(hipMemcpy(s->d_pl, s->pl.data(), sizeof(gcq_rt_plane) * s->pl.size(), hipMemcpyHostToDevice));
-  return GCQ_OK;
 }
 
 // Per-tile queues able to hold every pixel of a tile at once.
@@ -478,6 +275,40 @@ void set3(float* d, float x, float y, float z) {
 }
 
 }  // namespace
+
+int rtk::scene_upload(gcq_rt_scene* s) {
+  if (s->stream) return GCQ_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    gcqh::set_error("no HIP device visible");
+    return GCQ_ENODEV;
+  }
+  GCQ_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  GCQ_HIP(hipMalloc(&s->d_sph, sizeof(gcq_rt_sphere) * std::max<size_t>(1, s->sph.size())));
+  GCQ_HIP(hipMalloc(&s->d_pl, sizeof(gcq_rt_plane) * std::max<size_t>(1, s->pl.size())));
+  if (!s->sph.empty())
+    GCQ_HIP(hipMemcpy(s->d_sph, s->sph.data(), sizeof(gcq_rt_sphere) * s->sph.size(), hipMemcpyHostToDevice));
+  if (!s->pl.empty())
+    GCQ_HIP(hipMemcpy(s->d_pl, s->pl.data(), sizeof(gcq_rt_plane) * s->pl.size(), hipMemcpyHostToDevice));
+  return GCQ_OK;
+}
+
+rtk::Scene rtk::scene_view(const gcq_rt_scene* s) {
+  rtk::Scene sc;
+  sc.sph = s->d_sph;
+  sc.pl = s->d_pl;
+  sc.ns = (int32_t)s->sph.size();
+  sc.np = (int32_t)s->pl.size();
+  const gcq_rt_scene_desc& d = s->desc;
+  auto hn = [](const float* v) { return rtk::norm(rtk::v3(v[0], v[1], v[2])); };
+  sc.cam_o = rtk::v3(d.cam_o[0], d.cam_o[1], d.cam_o[2]);
+  sc.cam_f = hn(d.cam_f);  // rt_final.cpp:851-859 normalises on the host
+  sc.cam_r = hn(d.cam_r);
+  sc.cam_u = hn(d.cam_u);
+  sc.light_raw = rtk::v3(d.light[0], d.light[1], d.light[2]);
+  sc.ambient = rtk::v3(d.ambient[0], d.ambient[1], d.ambient[2]);
+  return sc;
+}
 
 extern "C" void gcq_rt_config_default(gcq_rt_config* c) {
   if (!c) return;
@@ -619,7 +450,7 @@ extern "C" int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* i
               "region outside the image");
   GCQ_REQUIRE((int64_t)cfg->tile_w * cfg->tile_h < (1ll << 30), "tile too large");
   std::memset(res, 0, sizeof *res);
-  int rc = ensure_device(s);
+  int rc = rtk::scene_upload(s);
   if (rc) return rc;
   const auto t0 = std::chrono::steady_clock::now();
   const int32_t tiles_x = (w + cfg->tile_w - 1) / cfg->tile_w, tiles_y = (h + cfg->tile_h - 1) / cfg->tile_h;
@@ -647,18 +478,7 @@ extern "C" int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* i
   }
   if (!s->err) GCQ_HIP(hipMalloc(&s->err, 8));
 
-  rtk::Scene sc;
-  sc.sph = s->d_sph;
-  sc.pl = s->d_pl;
-  sc.ns = (int32_t)s->sph.size();
-  sc.np = (int32_t)s->pl.size();
-  const gcq_rt_scene_desc& d = s->desc;
-  sc.cam_o = rtk::v3(d.cam_o[0], d.cam_o[1], d.cam_o[2]);
-  sc.cam_f = host_norm(d.cam_f);  // rt_final.cpp:851-857 normalises on the host
-  sc.cam_r = host_norm(d.cam_r);
-  sc.cam_u = host_norm(d.cam_u);
-  sc.light_raw = rtk::v3(d.light[0], d.light[1], d.light[2]);
-  sc.ambient = rtk::v3(d.ambient[0], d.ambient[1], d.ambient[2]);
+  const rtk::Scene sc = rtk::scene_view(s);
   rtk::Job jb;
   jb.W = cfg->width;
   jb.H = cfg->height;

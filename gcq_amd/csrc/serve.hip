@@ -1,0 +1,903 @@
+// serve.hip — continuously batched serving of BFS queries and ray-traced
+// tile jobs in one persistent kernel.
+//
+// No counterpart in the reference (its harnesses run one workload per
+// process).  Per request the results are those of the plain paths: a BFS
+// query's depths are exact BFS depths (bfs_bench.cpp:327-633 semantics,
+// shared expansion in bfs_batch.hpp) and a tile's pixels follow
+// rt_final.cpp's image (rt_shade.hpp), so each result is the same alone and
+// in the mix.
+//
+// Design (MI355X):
+//  * One launch runs the whole service.  Its first bfs_blocks blocks form a
+//    BFS sub-grid: every admitted query advances one level per grid barrier
+//    through a shared (slot, vertex) frontier queue, as in batch.hip.  The
+//    remaining blocks form an RT sub-grid: each wave picks an active tile
+//    job, claims 64 primary rays at a time or pops a live bounce ray from
+//    the job's queue, and traces one bounce.
+//  * Requests arrive through a submission ring in pinned host memory: the
+//    host writes an entry and publishes its sequence word; the last block
+//    to reach each BFS barrier reads new entries (system-scope loads) and
+//    installs them — BFS queries into a slot whose depth array the sub-grid
+//    clears during the next level and whose source enters the frontier for
+//    the level after; tile jobs into an RT job slot that the RT waves pick
+//    up at once.  Completions go back through a pinned completion ring.
+//  * The host admission loop decides when a request may enter: a free slot,
+//    room in the device-memory budget (depth arrays, ray records, images)
+//    and in the frontier ring (the sum of V of in-flight queries).
+//  * Nothing is launched while the server runs: the host only reads and
+//    writes pinned memory, so no stream can end up queued behind the
+//    persistent kernel on a shared hardware queue.
+#pragma clang fp contract(off)
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+#include "bfs_batch.hpp"
+#include "graph_handle.hpp"
+#include "rt_handle.hpp"
+
+namespace sv {
+
+using namespace gcqf;
+
+constexpr uint32_t kMaxAdmit = 32;           // entries installed per BFS barrier
+constexpr uint64_t kRtIdleLimit = 1ull << 30;  // idle polls of an RT wave before it gives up
+constexpr uint32_t kMaxTilePixels = 1u << 16;  // RT items carry a 16-bit pixel and a 15-bit job generation
+
+// Submission ring entry (pinned host memory, written by the host; seq last).
+struct alignas(64) Req {
+  uint64_t seq;  // i + 1 once entry i is complete
+  uint64_t id;   // server-internal request number
+  int32_t type, graph, src, slot;
+  int32_t x0, y0, w, h;
+  uint64_t buf, buf2;  // BFS: depth array; RT: image, ray records (device arena)
+};
+static_assert(sizeof(Req) == 64, "req");
+
+// Completion ring entry (pinned host memory, written by the kernel; seq last).
+struct alignas(64) Comp {
+  uint64_t seq;
+  uint64_t id;
+  int32_t type, status, levels, slot;
+  uint64_t rays;
+  uint64_t pad[3];
+};
+static_assert(sizeof(Comp) == 64, "comp");
+
+struct GraphDev {
+  uint64_t off, idx;
+  uint32_t nv, pad;
+  uint64_t pad2;
+};
+
+// One RT job slot; the three counters sit on their own 128-B lines.
+struct RtJob {
+  uint64_t img, recs, id;
+  int32_t x0, y0, w, h;
+  uint32_t gen, npx, active, pad;
+  uint64_t p0[9];
+  uint64_t next_primary, p1[15];  // gen << 32 | next unclaimed primary ray
+  uint64_t alive, p2[15];         // live rays
+  uint64_t traced, p3[15];
+};
+static_assert(sizeof(RtJob) == 512, "rt job");
+
+struct Ctl {
+  Barrier bar;
+  uint32_t err, p1[31];  // 1 frontier enqueue, 2 RT enqueue, 4 completion ring full, 8 RT idle limit
+  uint32_t exit, p2[31];
+  uint32_t adm_n, adm_slot[kMaxAdmit], p3[31];
+  uint64_t sub_head, p4[15];
+  uint64_t comp_tail, p5[15];
+  int32_t active_bfs, p6[31];
+  int32_t active_rt, p7[31];
+  int32_t levels, p8[31];
+};
+
+struct Dev {
+  bk::Dev bd;              // read side of the BFS slot table
+  bk::Slot* slots;         // the same table, written at admission
+  uint32_t* slot_active;   // per BFS slot
+  const GraphDev* graphs;
+  RtJob* jobs;
+  char* rings;             // one G-LFQ ring per RT job slot
+  uint64_t ring_stride;
+  Req* sub;
+  Comp* comp;
+  const uint32_t* stop;    // set by the host once every request has completed
+  Ctl* c;
+  rtk::Scene sc;
+  int32_t W, H, max_depth;
+  uint32_t njobs, bfs_blocks, max_requests;
+};
+
+__device__ void publish(const Dev& d, uint64_t id, int32_t type, int32_t levels, int32_t slot, uint64_t rays) {
+  const uint64_t k = gcq::faa_rlx(&d.c->comp_tail, (uint64_t)1);
+  if (k >= d.max_requests) {
+    gcq::or_rlx(&d.c->err, 4u);
+    return;
+  }
+  Comp* e = &d.comp[k];
+  gcq::st_sys(&e->id, id);
+  gcq::st_sys(&e->type, type);
+  gcq::st_sys(&e->status, 0);
+  gcq::st_sys(&e->levels, levels);
+  gcq::st_sys(&e->slot, slot);
+  gcq::st_sys(&e->rays, rays);
+  gcq::vm_wait();  // the record before its sequence word
+  gcq::st_sys(&e->seq, k + 1);
+}
+
+// Install new requests (one thread: the last block to reach the barrier
+// that ends level L).
+__device__ void admit(const Dev& d, int32_t L) {
+  Ctl* c = d.c;
+  uint64_t h = gcq::ld_rlx(&c->sub_head);
+  uint32_t n = 0;
+  while (n < kMaxAdmit && h < d.max_requests) {
+    Req* e = &d.sub[h];
+    if (gcq::ld_sys(&e->seq) != h + 1) break;
+    gcq::vm_wait();  // the sequence word before the fields
+    const int32_t type = gcq::ld_sys(&e->type), slot = gcq::ld_sys(&e->slot);
+    const uint64_t id = gcq::ld_sys(&e->id), buf = gcq::ld_sys(&e->buf);
+    if (type == GCQ_REQ_BFS) {
+      const GraphDev& g = d.graphs[gcq::ld_sys(&e->graph)];
+      bk::Slot* s = &d.slots[slot];
+      gcq::st_rlx(&s->off, g.off);
+      gcq::st_rlx(&s->idx, g.idx);
+      gcq::st_rlx(&s->depth, buf);
+      gcq::st_rlx(&s->src, (uint32_t)gcq::ld_sys(&e->src));
+      gcq::st_rlx(&s->nv, g.nv);
+      gcq::st_rlx(&s->start, (uint32_t)(L + 2));  // cleared during L+1, source expanded at L+2
+      gcq::st_rlx(&s->pad2[0], id);
+      gcq::st_rlx(&d.bd.alive[slot * bk::kAliveStride], (uint32_t)(L + 2));
+      gcq::st_rlx(&d.slot_active[slot], 1u);
+      gcq::st_rlx(&c->adm_slot[n], (uint32_t)slot);
+      ++n;
+      gcq::faa_rlx(&c->active_bfs, 1);
+    } else {
+      RtJob* j = &d.jobs[slot];
+      const uint32_t gen = (gcq::ld_rlx(&j->gen) + 1u) & 0x7FFFu;
+      const int32_t w = gcq::ld_sys(&e->w), hh = gcq::ld_sys(&e->h);
+      gcq::st_rlx(&j->img, buf);
+      gcq::st_rlx(&j->recs, gcq::ld_sys(&e->buf2));
+      gcq::st_rlx(&j->id, id);
+      gcq::st_rlx(&j->x0, gcq::ld_sys(&e->x0));
+      gcq::st_rlx(&j->y0, gcq::ld_sys(&e->y0));
+      gcq::st_rlx(&j->w, w);
+      gcq::st_rlx(&j->h, hh);
+      gcq::st_rlx(&j->npx, (uint32_t)(w * hh));
+      gcq::st_rlx(&j->gen, gen);
+      gcq::st_rlx(&j->alive, (uint64_t)(w * hh));
+      gcq::st_rlx(&j->traced, (uint64_t)0);
+      gcq::vm_wait();  // fields before the claim counter that carries the generation
+      gcq::st_rlx(&j->next_primary, (uint64_t)gen << 32);
+      gcq::vm_wait();
+      gcq::st_rlx(&j->active, 1u);
+      gcq::faa_rlx(&c->active_rt, 1);
+    }
+    ++h;
+  }
+  gcq::st_rlx(&c->sub_head, h);
+  gcq::st_rlx(&c->adm_n, n);
+  if (n == 0 && gcq::ld_sys(d.stop) && gcq::ld_rlx(&c->active_bfs) == 0 && gcq::ld_rlx(&c->active_rt) == 0)
+    gcq::st_rlx(&c->exit, 1u);
+}
+
+// Block 0 after the barrier ending level L: queries with nothing discovered
+// at level L are finished.
+__device__ void done_scan(const Dev& d, int32_t L) {
+  for (uint32_t s = threadIdx.x; s < d.bd.nslots; s += blockDim.x) {
+    if (!gcq::ld_rlx(&d.slot_active[s])) continue;
+    const int32_t start = (int32_t)gcq::ld_rlx(&d.slots[s].start);
+    if (start > L || (int32_t)gcq::ld_rlx(&d.bd.alive[s * bk::kAliveStride]) >= L + 1) continue;
+    const uint64_t id = gcq::ld_rlx(&d.slots[s].pad2[0]);
+    gcq::st_rlx(&d.slot_active[s], 0u);
+    gcq::faa_rlx(&d.c->active_bfs, -1);
+    gcq::vm_wait();  // slot free and count down before the host can reuse the slot
+    publish(d, id, GCQ_REQ_BFS, L + 1 - start, (int32_t)s, 0);
+  }
+}
+
+template <class F>
+__device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)d.bfs_blocks * blockDim.x;
+  int32_t L = 0;
+  for (;; ++L) {
+    if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
+    // Queries installed at the previous barrier: clear their depth arrays
+    // (0 at the source) and put their sources into this level's output.
+    const uint32_t na = gcq::ld_rlx(&d.c->adm_n);
+    for (uint32_t a = 0; a < na; ++a) {
+      const uint32_t s = gcq::ld_rlx(&d.c->adm_slot[a]);
+      const bk::Slot* sl = &d.slots[s];
+      int32_t* dep = (int32_t*)gcq::ld_rlx(&sl->depth);
+      const uint32_t nv = gcq::ld_rlx(&sl->nv), src = gcq::ld_rlx(&sl->src);
+      for (uint64_t v = tid; v < nv; v += nthreads) gcq::st_rlx(&dep[v], v == src ? 0 : -1);
+      if (tid == 0 && !fq.out(L).push(tid, (s << d.bd.vbits) | src)) gcq::or_rlx(&d.c->err, 1u);
+    }
+    (void)expand(fq.in(L), fq.out(L), d.bd, L, tid, stage);
+    if (!grid_barrier(&d.c->bar, d.bfs_blocks, [&]() {
+          if (F::kQueues == 3) fq.idle(L).rewind();
+          admit(d, L);
+        }))
+      break;
+    if (gcq::ld_rlx(&d.c->exit)) break;
+    if (blockIdx.x == 0) done_scan(d, L);
+  }
+  if (tid == 0) d.c->levels = L + 1;
+}
+
+__device__ void rt_part(const Dev& d) {
+  const uint32_t lane = gcq::lane_id();
+  const uint32_t rtid = (blockIdx.x - d.bfs_blocks) * blockDim.x + threadIdx.x;
+  const rtk::V3 light = rtk::norm(d.sc.light_raw);
+  uint32_t j = (rtid / 64) % d.njobs;
+  uint64_t idle = 0;
+  for (;;) {
+    int32_t pick = -1;
+    for (uint32_t k = 0; k < d.njobs; ++k) {
+      const uint32_t jj = (j + k) % d.njobs;
+      if (gcq::ld_rlx(&d.jobs[jj].active)) {
+        pick = (int32_t)jj;
+        break;
+      }
+    }
+    if (pick < 0) {
+      if (gcq::ld_rlx(&d.c->exit)) break;
+      if (++idle > kRtIdleLimit) {
+        if (lane == 0) gcq::or_rlx(&d.c->err, 8u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+      continue;
+    }
+    idle = 0;
+    j = (uint32_t)pick;
+    RtJob* job = &d.jobs[j];
+    const FLfq q{(gcq::lfq::Ring*)(d.rings + j * d.ring_stride)};
+    // Claim up to 64 primaries; the counter carries the job generation, so
+    // a claim made after the slot was reused reloads the new job's fields.
+    uint64_t claim = ~0ull;
+    if (lane == 0) {
+      const uint64_t cur = gcq::ld_rlx(&job->next_primary);
+      if ((uint32_t)cur < gcq::ld_rlx(&job->npx)) claim = gcq::faa_rlx(&job->next_primary, (uint64_t)64);
+    }
+    claim = gcq::bcast64(claim, 0);
+    gcq::vm_wait();
+    uint32_t gen = gcq::ld_rlx(&job->gen);
+    if (claim != ~0ull) gen = (uint32_t)(claim >> 32);  // fields of that generation are already published
+    const uint32_t npx = gcq::ld_rlx(&job->npx);
+    const int32_t x0 = gcq::ld_rlx(&job->x0), y0 = gcq::ld_rlx(&job->y0), w = gcq::ld_rlx(&job->w);
+    float* img = (float*)gcq::ld_rlx(&job->img);
+    rtk::RayRec* recs = (rtk::RayRec*)gcq::ld_rlx(&job->recs);
+    const uint64_t jid = gcq::ld_rlx(&job->id);
+    bool have = false;
+    uint32_t i = 0;
+    rtk::Ray r;
+    if (claim != ~0ull && (uint32_t)claim + lane < npx) {
+      i = (uint32_t)claim + lane;
+      r = rtk::primary(d.sc, d.W, d.H, x0 + (int32_t)(i % (uint32_t)w), y0 + (int32_t)(i / (uint32_t)w));
+      have = true;
+    } else {
+      uint32_t item = 0;
+      if (q.pop(rtid, &item)) {
+        if ((item >> 16) == gen) {
+          i = item & 0xFFFFu;
+          r = rtk::load_ray(&recs[i]);
+          have = true;
+        } else if (!q.push(rtid, item)) {  // a newer job's ray: leave it for fresh fields
+          gcq::or_rlx(&d.c->err, 2u);
+        }
+      }
+    }
+    if (__ballot(have) == 0) {
+      j = (j + 1) % d.njobs;
+      __builtin_amdgcn_s_sleep(2);
+      continue;
+    }
+    uint32_t ended = 0;
+    if (have) {
+      if (rtk::shade(d.sc, d.max_depth, light, r)) {
+        rtk::store_ray(&recs[i], r);
+        gcq::vm_wait();  // record complete before its index is published
+        if (!q.push(rtid, i | (gen << 16))) {
+          gcq::or_rlx(&d.c->err, 2u);
+          ended = 1;
+        }
+      } else {
+        // Device-scope stores: this arena block may next hold another
+        // request's state, written and read from other XCDs while this
+        // kernel still runs; a plain store could linger dirty in this XCD's
+        // L2 and be written back over it later.
+        gcq::st_rlx(&img[3 * i + 0], r.acc.x);
+        gcq::st_rlx(&img[3 * i + 1], r.acc.y);
+        gcq::st_rlx(&img[3 * i + 2], r.acc.z);
+        ended = 1;
+      }
+    }
+    const uint64_t e = gcq::wave_sum((uint64_t)ended), t = gcq::wave_sum((uint64_t)(have ? 1u : 0u));
+    if (lane == 0) {
+      gcq::faa_rlx(&job->traced, t);
+      gcq::vm_wait();  // every wave's traced count lands before its alive decrement
+      if (e && gcq::faa_rlx(&job->alive, (uint64_t)0 - e) == e) {
+        const uint64_t rays = gcq::ld_rlx(&job->traced);
+        gcq::st_rlx(&job->active, 0u);
+        gcq::faa_rlx(&d.c->active_rt, -1);
+        gcq::vm_wait();  // slot free before the host can reuse it
+        publish(d, jid, GCQ_REQ_RT, 0, (int32_t)j, rays);
+      }
+    }
+  }
+}
+
+template <class F>
+__global__ void k_serve(Frontiers<F> fq, Dev d) {
+  __shared__ uint32_t stage[16][128];
+  if (blockIdx.x < d.bfs_blocks) bfs_part(fq, d, stage[(threadIdx.x / 64) & 15]);
+  else rt_part(d);
+}
+
+__global__ void k_init_rings(char* pool, uint64_t stride, uint32_t n, uint32_t nrings) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  for (uint32_t t = 0; t < nrings; ++t) gcq::lfq::init_slice((gcq::lfq::Ring*)(pool + t * stride), n, false, 0, gid, st, 0);
+}
+
+// First-fit allocator over the request-state arena, in 256-B units.
+struct Arena {
+  std::map<uint64_t, uint64_t> free_;  // offset -> length
+  void init(uint64_t size) {
+    free_.clear();
+    free_[0] = size;
+  }
+  bool alloc(uint64_t n, uint64_t* off) {
+    n = (n + 255) & ~255ull;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        const uint64_t rest = it->second - n, at = it->first + n;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    n = (n + 255) & ~255ull;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Pending {
+  gcq_request r;
+  int32_t index;  // position in a trace, or -1
+};
+struct InFlight {
+  gcq_request r;
+  int32_t index, slot;
+  uint64_t off1, len1, off2, len2;
+  uint32_t nv;
+  double admit_ms;
+};
+struct Kept {
+  int32_t type;
+  uint64_t off, bytes;
+};
+
+}  // namespace sv
+
+struct gcq_server {
+  std::vector<gcq_graph*> graphs;
+  gcq_rt_scene* scene = nullptr;
+  int32_t W = 0, H = 0;
+  gcq_server_config cfg{};
+  hipStream_t stream = nullptr;
+  gcqf::FrontierSet fs;
+  uint32_t vbits = 1;
+  // device
+  bk::Slot* slots = nullptr;
+  uint32_t* alive = nullptr;
+  uint32_t* slot_active = nullptr;
+  sv::GraphDev* dgraphs = nullptr;
+  sv::RtJob* jobs = nullptr;
+  char* rings = nullptr;
+  uint64_t ring_stride = 0;
+  sv::Ctl* ctl = nullptr;
+  char* arena_base = nullptr;  // in-flight request state, budget_bytes
+  sv::Arena arena;
+  char* keep_base = nullptr;   // kept results, keep_bytes
+  sv::Arena keep;
+  // pinned
+  sv::Req* sub = nullptr;
+  sv::Comp* comp = nullptr;
+  uint32_t* stop = nullptr;
+  // host state
+  bool running = false, stopped = false;
+  uint64_t sub_tail = 0, comp_head = 0;
+  std::vector<int32_t> free_bfs, free_rt;
+  uint64_t frontier_used = 0;
+  std::deque<sv::Pending> pending;
+  std::map<uint64_t, sv::InFlight> inflight;  // by internal id (submission index)
+  std::map<uint64_t, sv::Kept> kept;          // by caller id
+  std::deque<gcq_completion> ready;
+  std::vector<int32_t> ready_index;
+  std::chrono::steady_clock::time_point t0;
+  double now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+constexpr double kStopTimeoutMs = 30000.0;
+
+void release_all(gcq_server* s) {
+  for (void* p : {(void*)s->slots, (void*)s->alive, (void*)s->slot_active, (void*)s->dgraphs, (void*)s->jobs,
+                  (void*)s->rings, (void*)s->ctl, (void*)s->arena_base, (void*)s->keep_base})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)s->sub, (void*)s->comp, (void*)s->stop})
+    if (p) (void)hipHostFree(p);
+  s->fs.release();
+}
+
+uint64_t rt_bytes(int32_t w, int32_t h) { return (uint64_t)w * h * (12ull + sizeof(rtk::RayRec)); }
+
+int check_request(gcq_server* s, const gcq_request* r) {
+  GCQ_REQUIRE(!r->keep || s->keep_base, "kept results need keep_bytes > 0");
+  if (r->type == GCQ_REQ_BFS) {
+    GCQ_REQUIRE(r->graph >= 0 && r->graph < (int32_t)s->graphs.size(), "bad graph %d", r->graph);
+    const uint64_t nv = (uint64_t)s->graphs[r->graph]->host.nv;
+    GCQ_REQUIRE(4 * nv <= s->cfg.budget_bytes, "query needs %llu bytes, above the budget",
+                (unsigned long long)(4 * nv));
+    GCQ_REQUIRE(nv <= (1ull << s->cfg.frontier_log2), "graph larger than the frontier ring");
+  } else if (r->type == GCQ_REQ_RT) {
+    GCQ_REQUIRE(s->scene, "server has no scene");
+    GCQ_REQUIRE(r->w > 0 && r->h > 0 && (uint64_t)r->w * r->h <= sv::kMaxTilePixels, "tile must hold 1..65536 px");
+    GCQ_REQUIRE(r->x0 >= 0 && r->y0 >= 0 && r->x0 + r->w <= s->W && r->y0 + r->h <= s->H, "tile outside the frame");
+    GCQ_REQUIRE(rt_bytes(r->w, r->h) <= s->cfg.budget_bytes, "tile above the budget");
+  } else {
+    GCQ_REQUIRE(false, "unknown request type %d", r->type);
+  }
+  return GCQ_OK;
+}
+
+// Install pending requests, in arrival order, while a slot and memory allow.
+int try_admit(gcq_server* s) {
+  while (!s->pending.empty()) {
+    sv::Pending& p = s->pending.front();
+    const gcq_request& r = p.r;
+    GCQ_REQUIRE(s->sub_tail < s->cfg.max_requests, "server request limit (%u) reached", s->cfg.max_requests);
+    sv::InFlight f{};
+    f.r = r;
+    f.index = p.index;
+    sv::Req e{};
+    e.id = s->sub_tail;
+    e.type = r.type;
+    if (r.type == GCQ_REQ_BFS) {
+      gcq_graph* g = s->graphs[r.graph];
+      const uint32_t nv = (uint32_t)g->host.nv;
+      if (s->free_bfs.empty() || s->frontier_used + nv > (1ull << s->cfg.frontier_log2)) break;
+      sv::Arena& a1 = r.keep ? s->keep : s->arena;
+      if (!a1.alloc(4ull * nv, &f.off1)) {
+        GCQ_REQUIRE(!r.keep, "keep_bytes exhausted by kept results");
+        break;
+      }
+      f.len1 = 4ull * nv;
+      f.nv = nv;
+      f.slot = s->free_bfs.back();
+      s->free_bfs.pop_back();
+      s->frontier_used += nv;
+      e.graph = r.graph;
+      e.src = (r.src >= 0 && r.src < (int32_t)nv) ? r.src : 0;  // bfs_bench.cpp:555
+      e.buf = (uint64_t)((r.keep ? s->keep_base : s->arena_base) + f.off1);
+    } else {
+      if (s->free_rt.empty()) break;
+      const uint64_t img = 12ull * r.w * r.h, rec = (uint64_t)sizeof(rtk::RayRec) * r.w * r.h;
+      sv::Arena& a1 = r.keep ? s->keep : s->arena;
+      if (!a1.alloc(img, &f.off1)) {
+        GCQ_REQUIRE(!r.keep, "keep_bytes exhausted by kept results");
+        break;
+      }
+      if (!s->arena.alloc(rec, &f.off2)) {
+        a1.release(f.off1, img);
+        break;
+      }
+      f.len1 = img;
+      f.len2 = rec;
+      f.slot = s->free_rt.back();
+      s->free_rt.pop_back();
+      e.x0 = r.x0;
+      e.y0 = r.y0;
+      e.w = r.w;
+      e.h = r.h;
+      e.buf = (uint64_t)((r.keep ? s->keep_base : s->arena_base) + f.off1);
+      e.buf2 = (uint64_t)(s->arena_base + f.off2);
+    }
+    e.slot = f.slot;
+    f.admit_ms = s->now_ms();
+    sv::Req* dst = &s->sub[s->sub_tail];
+    std::memcpy((char*)dst + 8, (const char*)&e + 8, sizeof e - 8);
+    __atomic_store_n(&dst->seq, s->sub_tail + 1, __ATOMIC_RELEASE);
+    s->inflight[s->sub_tail] = f;
+    ++s->sub_tail;
+    s->pending.pop_front();
+  }
+  // Nothing in flight will free anything: the head can never be admitted.
+  GCQ_REQUIRE(s->pending.empty() || !s->inflight.empty(), "request %llu does not fit the server's slots or budget",
+              (unsigned long long)s->pending.front().r.id);
+  return GCQ_OK;
+}
+
+void collect(gcq_server* s) {
+  for (;;) {
+    sv::Comp* e = &s->comp[s->comp_head];
+    if (s->comp_head >= s->cfg.max_requests || __atomic_load_n(&e->seq, __ATOMIC_ACQUIRE) != s->comp_head + 1) break;
+    ++s->comp_head;
+    auto it = s->inflight.find(e->id);
+    if (it == s->inflight.end()) continue;
+    sv::InFlight& f = it->second;
+    gcq_completion c{};
+    c.id = f.r.id;
+    c.type = f.r.type;
+    c.status = e->status;
+    c.levels = e->levels;
+    c.rays = e->rays;
+    c.arrival_ms = f.r.arrival_ms;
+    c.admit_ms = f.admit_ms;
+    c.done_ms = s->now_ms();
+    if (f.r.type == GCQ_REQ_BFS) {
+      s->free_bfs.push_back(f.slot);
+      s->frontier_used -= f.nv;
+    } else {
+      s->free_rt.push_back(f.slot);
+      s->arena.release(f.off2, f.len2);
+    }
+    if (f.r.keep) s->kept[f.r.id] = sv::Kept{f.r.type, f.off1, f.len1};
+    else s->arena.release(f.off1, f.len1);  // kept results stay in the keep arena until destroy
+    s->ready.push_back(c);
+    s->ready_index.push_back(f.index);
+    s->inflight.erase(it);
+  }
+}
+
+int submit_at(gcq_server* s, const gcq_request* r, int32_t index) {
+  int rc = check_request(s, r);
+  if (rc) return rc;
+  s->pending.push_back(sv::Pending{*r, index});
+  return try_admit(s);
+}
+
+}  // namespace
+
+extern "C" void gcq_server_config_default(gcq_server_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof *c);
+  c->kind = GCQ_GLFQ;
+  c->bfs_threads = 65536;
+  c->rt_threads = 16384;
+  c->block = 256;
+  c->bfs_slots = 256;
+  c->rt_slots = 16;
+  c->frontier_log2 = 26;
+  c->max_depth = 4;
+  c->pop_width = 64;
+  c->flush_at = 64;
+  c->budget_bytes = 64ull << 30;
+  c->max_requests = 1u << 20;
+}
+
+extern "C" int gcq_server_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_rt_scene* scene, int32_t frame_w,
+                                 int32_t frame_h, const gcq_server_config* cfg, gcq_server** out) {
+  GCQ_REQUIRE(out && cfg && (graphs || ngraphs == 0) && ngraphs >= 0, "null argument");
+  GCQ_REQUIRE(ngraphs > 0 || scene, "a server needs graphs or a scene");
+  GCQ_REQUIRE(cfg->block >= 64 && cfg->block <= 1024 && cfg->block % 64 == 0, "bad block");
+  GCQ_REQUIRE(cfg->bfs_threads >= cfg->block && cfg->bfs_threads % cfg->block == 0, "bad bfs_threads");
+  GCQ_REQUIRE(cfg->rt_threads % cfg->block == 0, "bad rt_threads");
+  GCQ_REQUIRE(!scene || cfg->rt_threads > 0, "a scene needs rt_threads > 0");
+  GCQ_REQUIRE(cfg->bfs_slots >= 1 && cfg->rt_slots >= 1 && cfg->rt_slots <= 1024, "bad slot counts");
+  GCQ_REQUIRE(cfg->frontier_log2 >= 10 && cfg->frontier_log2 <= 30, "frontier_log2 must be in [10, 30]");
+  GCQ_REQUIRE(cfg->pop_width >= 1 && cfg->pop_width <= 64 && cfg->flush_at >= 1 && cfg->flush_at <= 64,
+              "bad pop_width / flush_at");
+  GCQ_REQUIRE(cfg->max_requests >= 1 && cfg->budget_bytes >= 4096 && cfg->max_depth >= 0, "bad limits");
+  GCQ_REQUIRE(!scene || (frame_w > 0 && frame_h > 0), "bad frame");
+  gcq_server* s = new gcq_server();
+  s->graphs.assign(graphs, graphs + ngraphs);
+  s->scene = scene;
+  s->W = frame_w;
+  s->H = frame_h;
+  s->cfg = *cfg;
+  int32_t max_nv = 1;
+  for (gcq_graph* g : s->graphs) {
+    if (!g) {
+      delete s;
+      GCQ_REQUIRE(false, "null graph");
+    }
+    max_nv = std::max(max_nv, g->host.nv);
+  }
+  while ((1ll << s->vbits) < (int64_t)max_nv) ++s->vbits;
+  if ((uint64_t)cfg->bfs_slots > (1ull << (31 - s->vbits))) {
+    delete s;
+    GCQ_REQUIRE(false, "%u BFS slots do not fit beside %u-bit vertices", cfg->bfs_slots, s->vbits);
+  }
+  *out = s;
+  return GCQ_OK;
+}
+
+extern "C" int gcq_server_start(gcq_server* s) {
+  GCQ_REQUIRE(s && !s->running && !s->stopped, "server already started");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    gcqh::set_error("no HIP device visible");
+    return GCQ_ENODEV;
+  }
+  const gcq_server_config& c = s->cfg;
+  for (gcq_graph* g : s->graphs) {
+    int rc = gcqg::ensure_uploaded(g);
+    if (rc) return rc;
+  }
+  if (s->scene) {
+    int rc = rtk::scene_upload(s->scene);
+    if (rc) return rc;
+  }
+  GCQ_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  int rc = s->fs.ensure(c.kind, 1ull << c.frontier_log2, c.bfs_threads);
+  if (rc) return rc;
+  const uint32_t S = c.bfs_slots, J = c.rt_slots;
+  GCQ_HIP(hipMalloc(&s->slots, sizeof(bk::Slot) * S));
+  GCQ_HIP(hipMalloc(&s->alive, 4ull * bk::kAliveStride * S));
+  GCQ_HIP(hipMalloc(&s->slot_active, 4ull * S));
+  GCQ_HIP(hipMalloc(&s->dgraphs, sizeof(sv::GraphDev) * std::max<size_t>(1, s->graphs.size())));
+  GCQ_HIP(hipMalloc(&s->jobs, sizeof(sv::RtJob) * J));
+  s->ring_stride = (gcq::lfq::ring_bytes(sv::kMaxTilePixels) + 255) & ~255ull;
+  GCQ_HIP(hipMalloc(&s->rings, s->ring_stride * J));
+  GCQ_HIP(hipMalloc(&s->ctl, sizeof(sv::Ctl)));
+  GCQ_HIP(hipMalloc(&s->arena_base, c.budget_bytes));
+  s->arena.init(c.budget_bytes);
+  if (c.keep_bytes) {
+    GCQ_HIP(hipMalloc(&s->keep_base, c.keep_bytes));
+    s->keep.init(c.keep_bytes);
+  }
+  GCQ_HIP(hipHostMalloc((void**)&s->sub, sizeof(sv::Req) * (size_t)c.max_requests, hipHostMallocCoherent));
+  GCQ_HIP(hipHostMalloc((void**)&s->comp, sizeof(sv::Comp) * (size_t)c.max_requests, hipHostMallocCoherent));
+  GCQ_HIP(hipHostMalloc((void**)&s->stop, 64, hipHostMallocCoherent));
+  std::memset(s->sub, 0, sizeof(sv::Req) * (size_t)c.max_requests);
+  std::memset(s->comp, 0, sizeof(sv::Comp) * (size_t)c.max_requests);
+  __atomic_store_n(s->stop, 0u, __ATOMIC_RELEASE);
+  std::vector<sv::GraphDev> gd(std::max<size_t>(1, s->graphs.size()));
+  for (size_t i = 0; i < s->graphs.size(); ++i)
+    gd[i] = sv::GraphDev{(uint64_t)s->graphs[i]->d_off, (uint64_t)s->graphs[i]->d_idx,
+                         (uint32_t)s->graphs[i]->host.nv, 0, 0};
+  GCQ_HIP(hipMemcpy(s->dgraphs, gd.data(), sizeof(sv::GraphDev) * gd.size(), hipMemcpyHostToDevice));
+  auto zero = [&](void* p, uint64_t bytes) {
+    hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(bytes / 8)), dim3(256), 0, s->stream, (uint64_t*)p,
+                       bytes / 8);
+  };
+  zero(s->slots, sizeof(bk::Slot) * S);
+  zero(s->alive, ((4ull * bk::kAliveStride * S) + 7) & ~7ull);
+  zero(s->slot_active, (4ull * S + 7) & ~7ull);
+  zero(s->jobs, sizeof(sv::RtJob) * J);
+  zero(s->ctl, sizeof(sv::Ctl));
+  s->fs.reset(s->stream);
+  hipLaunchKernelGGL(sv::k_init_rings, dim3(gcqf::grid_of(2ull * sv::kMaxTilePixels)), dim3(256), 0, s->stream,
+                     s->rings, s->ring_stride, sv::kMaxTilePixels, J);
+  GCQ_HIP(hipGetLastError());
+  GCQ_HIP(hipStreamSynchronize(s->stream));
+
+  sv::Dev d{};
+  d.bd = bk::Dev{s->slots, s->alive, nullptr, &s->ctl->err, S, s->vbits, c.pop_width, c.flush_at};
+  d.slots = s->slots;
+  d.slot_active = s->slot_active;
+  d.graphs = s->dgraphs;
+  d.jobs = s->jobs;
+  d.rings = s->rings;
+  d.ring_stride = s->ring_stride;
+  d.sub = s->sub;
+  d.comp = s->comp;
+  d.stop = s->stop;
+  d.c = s->ctl;
+  if (s->scene) d.sc = rtk::scene_view(s->scene);
+  d.W = s->W;
+  d.H = s->H;
+  d.max_depth = c.max_depth;
+  d.njobs = J;
+  d.bfs_blocks = c.bfs_threads / c.block;
+  d.max_requests = c.max_requests;
+  const uint32_t grid = (c.bfs_threads + c.rt_threads) / c.block;
+  int dev = 0, cus = 0, per_cu = 0;
+  GCQ_HIP(hipGetDevice(&dev));
+  GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  rc = s->fs.dispatch([&](const auto& fq) {
+    using F = std::decay_t<decltype(fq.q[0])>;
+    GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sv::k_serve<F>, c.block, 0));
+    if ((int64_t)grid > (int64_t)per_cu * cus) {
+      gcqh::set_error("server grid of %u blocks does not fit resident (%d)", grid, per_cu * cus);
+      return GCQ_EINVAL;
+    }
+    hipLaunchKernelGGL(sv::k_serve<F>, dim3(grid), dim3(c.block), 0, s->stream, fq, d);
+    GCQ_HIP(hipGetLastError());
+    return GCQ_OK;
+  });
+  if (rc) return rc;
+  s->free_bfs.clear();
+  for (int32_t i = (int32_t)S - 1; i >= 0; --i) s->free_bfs.push_back(i);
+  s->free_rt.clear();
+  for (int32_t i = (int32_t)J - 1; i >= 0; --i) s->free_rt.push_back(i);
+  s->t0 = std::chrono::steady_clock::now();
+  s->running = true;
+  return GCQ_OK;
+}
+
+extern "C" int gcq_server_submit(gcq_server* s, const gcq_request* r) {
+  GCQ_REQUIRE(s && r && s->running, "server not running");
+  return submit_at(s, r, -1);
+}
+
+extern "C" int gcq_server_poll(gcq_server* s, gcq_completion* out, int32_t max, int32_t* n) {
+  GCQ_REQUIRE(s && s->running, "server not running");
+  collect(s);
+  int rc = try_admit(s);
+  int32_t k = 0;
+  while (k < max && !s->ready.empty()) {
+    if (out) out[k] = s->ready.front();
+    s->ready.pop_front();
+    s->ready_index.erase(s->ready_index.begin());
+    ++k;
+  }
+  if (n) *n = k;
+  return rc;
+}
+
+extern "C" int gcq_server_stop(gcq_server* s) {
+  GCQ_REQUIRE(s, "null server");
+  if (!s->running) return GCQ_OK;
+  const double deadline = s->now_ms() + kStopTimeoutMs;
+  while (!s->pending.empty() || !s->inflight.empty()) {
+    collect(s);
+    int rc = try_admit(s);
+    if (rc) return rc;
+    if (s->now_ms() > deadline) {
+      gcqh::set_error("server: requests still in flight after %.0f ms", kStopTimeoutMs);
+      return GCQ_ETIMEOUT;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+  __atomic_store_n(s->stop, 1u, __ATOMIC_RELEASE);
+  for (;;) {
+    const hipError_t q = hipStreamQuery(s->stream);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) GCQ_HIP(q);
+    if (s->now_ms() > deadline) {
+      gcqh::set_error("server: persistent kernel did not end");
+      return GCQ_ETIMEOUT;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+  s->running = false;
+  s->stopped = true;
+  sv::Ctl h;
+  GCQ_HIP(hipMemcpy(&h, s->ctl, sizeof h, hipMemcpyDeviceToHost));
+  if (h.bar.abort) {
+    gcqh::set_error("server aborted: a BFS barrier wait exceeded its spin limit");
+    return GCQ_ETIMEOUT;
+  }
+  if (h.err) {
+    gcqh::set_error("server kernel reported error flags 0x%x", h.err);
+    return (h.err & 8u) ? GCQ_ETIMEOUT : GCQ_EFULL;
+  }
+  return GCQ_OK;
+}
+
+extern "C" int gcq_server_fetch(gcq_server* s, uint64_t id, void* out) {
+  GCQ_REQUIRE(s && out, "null argument");
+  GCQ_REQUIRE(s->stopped, "results can be fetched once the server is stopped");
+  auto it = s->kept.find(id);
+  GCQ_REQUIRE(it != s->kept.end(), "no kept result for request %llu", (unsigned long long)id);
+  GCQ_HIP(hipMemcpy(out, s->keep_base + it->second.off, it->second.bytes, hipMemcpyDeviceToHost));
+  return GCQ_OK;
+}
+
+extern "C" int gcq_server_run_trace(gcq_server* s, const gcq_request* reqs, int32_t n, gcq_completion* out) {
+  GCQ_REQUIRE(s && reqs && out && n > 0, "null argument");
+  for (int32_t i = 1; i < n; ++i)
+    GCQ_REQUIRE(reqs[i].arrival_ms >= reqs[i - 1].arrival_ms, "trace must be sorted by arrival");
+  int rc = gcq_server_start(s);
+  if (rc) return rc;
+  int32_t next = 0, done = 0;
+  const double deadline = reqs[n - 1].arrival_ms + kStopTimeoutMs;
+  while (done < n) {
+    const double now = s->now_ms();
+    while (next < n && reqs[next].arrival_ms <= now) {
+      rc = submit_at(s, &reqs[next], next);
+      if (rc) return rc;
+      ++next;
+    }
+    collect(s);
+    rc = try_admit(s);
+    if (rc) return rc;
+    while (!s->ready.empty()) {
+      out[s->ready_index.front()] = s->ready.front();
+      s->ready.pop_front();
+      s->ready_index.erase(s->ready_index.begin());
+      ++done;
+    }
+    if (s->now_ms() > deadline) {
+      gcqh::set_error("trace did not finish");
+      return GCQ_ETIMEOUT;
+    }
+    if (next < n && reqs[next].arrival_ms > s->now_ms() + 0.05) std::this_thread::yield();
+  }
+  return gcq_server_stop(s);
+}
+
+extern "C" int gcq_server_destroy(gcq_server* s) {
+  if (!s) return GCQ_OK;
+  if (s->running) {
+    int rc = gcq_server_stop(s);
+    if (rc == GCQ_ETIMEOUT) return rc;  // the kernel still runs: do not free what it uses
+  }
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  release_all(s);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return GCQ_OK;
+}
+
+extern "C" int gcq_plain_run_trace(gcq_graph* const* graphs, int32_t ngraphs, gcq_rt_scene* scene, int32_t frame_w,
+                                   int32_t frame_h, const gcq_bfs_config* bcfg, const gcq_rt_config* rcfg,
+                                   const gcq_request* reqs, int32_t n, gcq_completion* out) {
+  GCQ_REQUIRE(reqs && out && n > 0 && bcfg && rcfg, "null argument");
+  const auto t0 = std::chrono::steady_clock::now();
+  auto now = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+  for (int32_t i = 0; i < n; ++i) {
+    const gcq_request& r = reqs[i];
+    while (now() < r.arrival_ms) std::this_thread::yield();
+    gcq_completion& c = out[i];
+    std::memset(&c, 0, sizeof c);
+    c.id = r.id;
+    c.type = r.type;
+    c.arrival_ms = r.arrival_ms;
+    c.admit_ms = now();
+    if (r.type == GCQ_REQ_BFS) {
+      GCQ_REQUIRE(r.graph >= 0 && r.graph < ngraphs, "bad graph %d", r.graph);
+      gcq_bfs_result res;
+      int rc = gcq_bfs_run(graphs[r.graph], r.src, bcfg, nullptr, &res);
+      if (rc) return rc;
+      c.levels = res.levels;
+    } else {
+      GCQ_REQUIRE(scene, "no scene");
+      gcq_rt_config rc2 = *rcfg;
+      rc2.width = frame_w;
+      rc2.height = frame_h;
+      rc2.x0 = r.x0;
+      rc2.y0 = r.y0;
+      rc2.w = r.w;
+      rc2.h = r.h;
+      gcq_rt_result res;
+      int rc = gcq_rt_render(scene, &rc2, nullptr, &res);
+      if (rc) return rc;
+      c.rays = res.total_rays;
+    }
+    c.done_ms = now();
+  }
+  return GCQ_OK;
+}

@@ -49,6 +49,25 @@ class RtResult(C.Structure):
                 ("total_ms", C.c_double), ("tiles", C.c_int32), ("pad", C.c_int32)]
 
 
+class Request(C.Structure):
+    _fields_ = [("type", C.c_int32), ("graph", C.c_int32), ("src", C.c_int32), ("x0", C.c_int32), ("y0", C.c_int32),
+                ("w", C.c_int32), ("h", C.c_int32), ("keep", C.c_int32), ("id", C.c_uint64), ("arrival_ms", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("type", C.c_int32), ("status", C.c_int32), ("levels", C.c_int32),
+                ("pad", C.c_int32), ("rays", C.c_uint64), ("arrival_ms", C.c_double), ("admit_ms", C.c_double),
+                ("done_ms", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("bfs_threads", C.c_uint32), ("rt_threads", C.c_uint32), ("block", C.c_uint32),
+                ("bfs_slots", C.c_uint32), ("rt_slots", C.c_uint32), ("frontier_log2", C.c_uint32),
+                ("max_depth", C.c_int32), ("pop_width", C.c_uint32), ("flush_at", C.c_uint32),
+                ("budget_bytes", C.c_uint64), ("keep_bytes", C.c_uint64), ("max_requests", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
 _P = C.POINTER
 VP = C.c_void_p
 
@@ -73,6 +92,17 @@ SIGNATURES = {
     "gcq_rt_scene_destroy": (C.c_int, [VP]),
     "gcq_rt_config_default": (None, [_P(RtConfig)]),
     "gcq_rt_render": (C.c_int, [VP, _P(RtConfig), VP, _P(RtResult)]),
+    "gcq_server_config_default": (None, [_P(ServerConfig)]),
+    "gcq_server_create": (C.c_int, [_P(VP), C.c_int32, VP, C.c_int32, C.c_int32, _P(ServerConfig), _P(VP)]),
+    "gcq_server_start": (C.c_int, [VP]),
+    "gcq_server_submit": (C.c_int, [VP, _P(Request)]),
+    "gcq_server_poll": (C.c_int, [VP, _P(Completion), C.c_int32, _P(C.c_int32)]),
+    "gcq_server_stop": (C.c_int, [VP]),
+    "gcq_server_fetch": (C.c_int, [VP, C.c_uint64, VP]),
+    "gcq_server_run_trace": (C.c_int, [VP, _P(Request), C.c_int32, _P(Completion)]),
+    "gcq_server_destroy": (C.c_int, [VP]),
+    "gcq_plain_run_trace": (C.c_int, [_P(VP), C.c_int32, VP, C.c_int32, C.c_int32, _P(BfsConfig), _P(RtConfig),
+                                      _P(Request), C.c_int32, _P(Completion)]),
 }
 
 

@@ -308,6 +308,74 @@ void gcq_rt_config_default(gcq_rt_config* cfg);
 /* Render cfg's region into img_out (w * h RGB float triples, row-major). */
 int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* img_out, gcq_rt_result* res);
 
+/* ------------------------------------------------------------------ serving
+ * No counterpart in the reference.  A continuously batched server: BFS
+ * queries on resident graphs and RT tile jobs on a resident scene are
+ * admitted by a host loop as slots and device memory free up and run in one
+ * persistent kernel (a BFS sub-grid advancing every admitted query level by
+ * level, an RT sub-grid tracing admitted tiles).  Requests reach the kernel
+ * through a submission ring in pinned host memory that it polls at every
+ * level barrier; completions come back through a pinned completion ring.
+ * While the server runs nothing else is launched, so no stream can queue
+ * behind the persistent kernel. */
+typedef struct gcq_server gcq_server;
+enum gcq_req_type { GCQ_REQ_BFS = 0, GCQ_REQ_RT = 1 };
+typedef struct gcq_request {
+  int32_t type;           /* enum gcq_req_type */
+  int32_t graph;          /* BFS: index into the server's graphs */
+  int32_t src;            /* BFS: source vertex (out of range -> 0) */
+  int32_t x0, y0, w, h;   /* RT: tile of the server's frame */
+  int32_t keep;           /* 1: keep the result (in keep_bytes) for gcq_server_fetch after stop */
+  uint64_t id;            /* caller's id, echoed in the completion */
+  double arrival_ms;      /* trace runs: arrival time after the trace start */
+} gcq_request;
+typedef struct gcq_completion {
+  uint64_t id;
+  int32_t type, status;   /* status 0 = done */
+  int32_t levels;         /* BFS: levels of the traversal */
+  int32_t pad;
+  uint64_t rays;          /* RT: rays traced */
+  double arrival_ms, admit_ms, done_ms;  /* host clock, after the trace start */
+} gcq_completion;
+typedef struct gcq_server_config {
+  int32_t kind;            /* BFS frontier queue (enum gcq_kind) */
+  uint32_t bfs_threads;    /* BFS sub-grid, multiple of block */
+  uint32_t rt_threads;     /* RT sub-grid, multiple of block */
+  uint32_t block;
+  uint32_t bfs_slots;      /* concurrent BFS queries */
+  uint32_t rt_slots;       /* concurrent RT tile jobs */
+  uint32_t frontier_log2;  /* frontier ring size; in-flight queries' V sum stays below it */
+  int32_t max_depth;       /* RT bounces */
+  uint32_t pop_width, flush_at;
+  uint64_t budget_bytes;   /* device memory for request state: depth arrays, ray records, images */
+  uint64_t keep_bytes;     /* separate device memory for results kept for gcq_server_fetch */
+  uint32_t max_requests;   /* submission / completion ring entries for the server's lifetime */
+  uint32_t pad;
+} gcq_server_config;
+void gcq_server_config_default(gcq_server_config* cfg);
+/* graphs and scene are borrowed and must outlive the server; scene may be
+ * NULL when no RT request will be sent. */
+int gcq_server_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_rt_scene* scene, int32_t frame_w,
+                      int32_t frame_h, const gcq_server_config* cfg, gcq_server** out);
+int gcq_server_start(gcq_server* s);
+/* Queue a request on the host; it is admitted as soon as a slot and memory allow. */
+int gcq_server_submit(gcq_server* s, const gcq_request* r);
+/* Admit what fits and collect up to max completions. */
+int gcq_server_poll(gcq_server* s, gcq_completion* out, int32_t max, int32_t* n);
+/* Wait for every submitted request, then end the persistent kernel. */
+int gcq_server_stop(gcq_server* s);
+/* After stop: the depth array (BFS, nv int32) or image (RT, w*h RGB floats) of a kept request. */
+int gcq_server_fetch(gcq_server* s, uint64_t id, void* out);
+/* Start, submit reqs[i] at reqs[i].arrival_ms, poll until all are done, stop;
+ * out[i] is the completion of reqs[i]. */
+int gcq_server_run_trace(gcq_server* s, const gcq_request* reqs, int32_t n, gcq_completion* out);
+int gcq_server_destroy(gcq_server* s);
+/* The plain comparator: the same trace served one request at a time with
+ * one launch (gcq_bfs_run / gcq_rt_render) per request. */
+int gcq_plain_run_trace(gcq_graph* const* graphs, int32_t ngraphs, gcq_rt_scene* scene, int32_t frame_w,
+                        int32_t frame_h, const gcq_bfs_config* bcfg, const gcq_rt_config* rcfg,
+                        const gcq_request* reqs, int32_t n, gcq_completion* out);
+
 #ifdef __cplusplus
 }
 #endif
