@@ -14,6 +14,21 @@ Beside it, measured in the same run:
     wave-aggregated FAA peak P1 (tickets/s on one 64-bit counter, same grid);
   * roofline_hbm: algorithmic bytes per launch / mean launch time vs 8 TB/s;
   * cpu_baseline: the oracle's host std::atomic G-WFQ on the host cores.
+
+Unless --headline-only, the line also carries the other workloads of the
+metric, each measured in this run ("secondary"):
+  * hbm_copy: measured streaming-copy bandwidth (the measured HBM roofline);
+  * bfs_lattice: queue-driven BFS on the 4900 x 4900 lattice (keep 0.75, 1%
+    diagonals, seed 7, source 0, 8,192 threads, G-LFQ, one launch), 3 warmup
+    + 10 timed runs: median MTEPS and algorithmic GB/s (28 B per reached
+    vertex + 8 B per scanned edge) against HBM, beside the serial CPU BFS;
+  * batched_bfs: 256 R-MAT queries (scales 12..20, seed 11) in one batched
+    launch (replayed hipGraph) against 256 plain one-launch-per-query runs;
+  * served: a Poisson trace (90% BFS on R-MAT scales 14..22, 10% 256x256
+    ray-traced tiles of the 64-sphere scene, 64 GB budget) through the
+    continuously batched server and through plain per-request launches:
+    p50 / p99 latency and throughput, plus both capacities on a burst;
+  * rt: the 1920 x 1080 64-sphere frame, 4 bounces, 256 x 256 tiles: Mrays/s.
 """
 import argparse
 import json
@@ -62,6 +77,7 @@ def main():
     ap.add_argument("--queue", default="gwfq", choices=["gwfq", "glfq", "wfq", "sfq"])
     ap.add_argument("--cpu-baseline-ms", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the BFS / batched / served / RT lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -122,7 +138,121 @@ def main():
         # the host share of one GPU is 16 cores whatever os.cpu_count() reports
         cores = min(len(os.sched_getaffinity(0)), 16)
         line["cpu_baseline"] = cpu_baseline(kind, cores, args.cpu_baseline_ms)
+    if not args.headline_only:
+        line["secondary"] = secondary(g, not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def secondary(g, with_cpu):
+    import numpy as np
+
+    from gcq_amd import bfs, rt, serve
+    out = {}
+    gbps, copy_ms = g.hbm_copy(1 << 30, 10)
+    out["hbm_copy"] = {"gbps": round(gbps, 1), "bytes": 1 << 30, "kernel_ms": round(copy_ms, 4)}
+    _progress(f"hbm copy {gbps:.0f} GB/s")
+
+    # --- plain BFS on the lattice
+    with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as gr:
+        nv, ne = gr.info()
+        for _ in range(3):
+            gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=bfs.PERSISTENT, depths=False)
+        ms = []
+        for _ in range(10):
+            _, res = gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=bfs.PERSISTENT, depths=False)
+            ms.append(res["kernel_ms"])
+        med = float(np.median(ms))
+        alg_bytes = 28 * res["reached"] + 8 * res["edges_scanned"]
+        o = {"workload": "lattice 4900x4900, keep 0.75, diag 0.01, seed 7, src 0, G-LFQ, 8192 threads, persistent",
+             "vertices": nv, "edges": ne, "levels": res["levels"], "reached": res["reached"],
+             "edges_scanned": res["edges_scanned"], "kernel_ms_median": round(med, 3),
+             "mteps": round(res["edges_scanned"] / med / 1e3, 1),
+             "roofline": {"bound": "hbm", "achieved": round(alg_bytes / med / 1e6, 2), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(alg_bytes / med / 1e6 / HBM_PEAK_GBS, 5),
+                          "bytes_per_launch": alg_bytes}}
+        if with_cpu:
+            from oracle import pyoracle as orc
+            off, idx = gr.csr()
+            t = time.perf_counter()
+            _, _, es = orc.bfs_serial(off, idx, 0)
+            cpu_s = time.perf_counter() - t
+            o["cpu_baseline"] = {"value": round(es / cpu_s / 1e6, 2), "unit": "MTEPS", "cores": 1, "kind": "port",
+                                 "sample": "one serial textbook BFS (oracle) on the same CSR"}
+    out["bfs_lattice"] = o
+    _progress(f"lattice BFS {o['mteps']} MTEPS")
+
+    # --- batched vs plain (256 R-MAT queries)
+    graphs, gids, srcs = bfs.rmat_query_set()
+    for gr in graphs:
+        gr.bfs(0, kind=g.GLFQ, threads=8192, depths=False)
+    t = time.perf_counter()
+    for gi, sv in zip(gids, srcs):
+        graphs[gi].bfs(int(sv), kind=g.GLFQ, threads=8192, depths=False)
+    plain_s = time.perf_counter() - t
+    with bfs.Batch(graphs) as b:
+        cfg = bfs.batch_config(threads=262144, block=512)
+        b.run(gids, srcs, cfg=cfg)
+        ts = []
+        for _ in range(5):
+            _, _, res = b.run(gids, srcs, cfg=cfg)
+            ts.append(res["total_ms"])
+        bms = float(np.median(ts))
+    out["batched_bfs"] = {
+        "workload": "256 R-MAT queries, scales 12..20 uniform, ef 16, seed 11, G-LFQ",
+        "queries_per_s": round(256 / bms * 1e3, 1), "batch_ms_median": round(bms, 2),
+        "plain_queries_per_s": round(256 / plain_s, 1), "vs_plain": round(plain_s * 1e3 / bms, 2),
+        "config": {k: getattr(cfg, k) for k in ("threads", "block", "pop_width", "flush_at", "use_graph")},
+        "replayed_graph": bool(res["graph_replayed"])}
+    for gr in graphs:
+        gr.close()
+    _progress(f"batched {out['batched_bfs']['queries_per_s']} q/s vs plain {out['batched_bfs']['plain_queries_per_s']}")
+
+    # --- served vs plain (Poisson mix)
+    graphs = [bfs.Graph.rmat(sc, 16, 0.57, 0.19, 0.19, 11) for sc in range(14, 23)]
+    scene = rt.Scene.preset(rt.RANDOM, 64, 3)
+    frame = (1920, 1080)
+    nvs = [gr.info()[0] for gr in graphs]
+    pick = lambda gi, i: graphs[gi].pick_source(11, i)  # noqa: E731
+    for gr in graphs:
+        gr.bfs(0, kind=g.GLFQ, threads=32768, depths=False)
+    lam = 150.0
+    rows = serve.poisson_trace(300, lam, nvs, frame, rt_fraction=0.1, seed=5, graph_sources=pick)
+    burst = serve.poisson_trace(300, 1e9, nvs, frame, rt_fraction=0.1, seed=6, graph_sources=pick)
+    scfg = dict(bfs_threads=196608, block=512, rt_threads=16384)
+    with serve.Server(graphs, scene, frame, **scfg) as srv:
+        s_trace = serve.latency_summary(srv.run_trace(rows))
+    with serve.Server(graphs, scene, frame, **scfg) as srv:
+        s_burst = serve.latency_summary(srv.run_trace(burst))
+    p_trace = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows, bfs_threads=32768))
+    p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, bfs_threads=32768))
+    rnd = lambda d: {k: round(v, 3) if isinstance(v, float) else v for k, v in d.items()}  # noqa: E731
+    out["served"] = {
+        "workload": f"Poisson {lam:.0f} req/s, 300 requests: 90% BFS on R-MAT scales 14..22, 10% 256x256 tiles "
+                    "of the 1920x1080 64-sphere scene (4 bounces), 64 GB budget, arrival seed 5",
+        "served": rnd(s_trace), "plain": rnd(p_trace),
+        "burst_served": rnd(s_burst), "burst_plain": rnd(p_burst),
+        "capacity_vs_plain": round(s_burst["throughput_rps"] / p_burst["throughput_rps"], 2),
+        "config": dict(scfg, plain_bfs_threads=32768, plain_rt_threads_per_tile=1024)}
+    for gr in graphs:
+        gr.close()
+    _progress(f"served p99 {s_trace['p99_ms']:.1f} ms vs plain {p_trace['p99_ms']:.1f} ms")
+
+    # --- ray tracing
+    scene.render(width=1920, height=1080, tile_w=256, tile_h=256)
+    ms = []
+    for _ in range(5):
+        _, res = scene.render(width=1920, height=1080, tile_w=256, tile_h=256)
+        ms.append(res["kernel_ms"])
+    med = float(np.median(ms))
+    out["rt"] = {"workload": "1920x1080, 64 random spheres + ground, seed 3, 4 bounces, 256x256 tiles, G-LFQ",
+                 "rays": res["total_rays"], "kernel_ms_median": round(med, 3),
+                 "mrays_per_s": round(res["total_rays"] / med / 1e3, 1)}
+    scene.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -104,6 +104,7 @@ def load():
         "gcq_queue_bytes": (C.c_uint64, [C.c_void_p]),
         "gcq_faa_peak": (C.c_int, [_P(FaaConfig), _P(FaaResult)]),
         "gcq_fence_cost": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P(C.c_double)]),
+        "gcq_hbm_copy": (C.c_int, [C.c_uint64, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
         "gcq_tb_run": (C.c_int, [C.c_void_p, _P(TbConfig), _P(TbResult)]),
         "gcq_fifo_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _P(FifoResult)]),
         "gcq_hist_run": (C.c_int, [C.c_void_p, _P(HistConfig), C.c_void_p, C.c_size_t, _P(C.c_size_t)]),
@@ -241,3 +242,10 @@ def fence_cost(mode, threads=65536, block=256, iters=512):
     ns = C.c_double(0)
     _check(load().gcq_fence_cost(threads, block, mode, iters, C.byref(ns)), "gcq_fence_cost")
     return ns.value
+
+
+def hbm_copy(nbytes=1 << 30, iters=20):
+    """Measured streaming-copy bandwidth (GB/s, read + write) and launch time."""
+    gbps, ms = C.c_double(0), C.c_double(0)
+    _check(load().gcq_hbm_copy(nbytes, iters, C.byref(gbps), C.byref(ms)), "gcq_hbm_copy")
+    return gbps.value, ms.value

@@ -315,6 +315,52 @@ extern "C" int gcq_faa_peak(const gcq_faa_config* cfg, gcq_faa_result* res) {
   return GCQ_OK;
 }
 
+namespace {
+// Streaming copy, 16 B per lane per iteration, grid-stride: the measured
+// HBM roofline (and the known-size kernel that calibrates HBM byte counters).
+__global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = gid; i < n; i += st) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int gcq_hbm_copy(uint64_t bytes, uint32_t iters, double* gbps, double* kernel_ms) {
+  GCQ_REQUIRE(bytes >= 4096 && bytes % 16 == 0 && iters >= 1 && gbps, "bad copy size");
+  int dev = 0, cus = 0;
+  GCQ_HIP(hipGetDevice(&dev));
+  GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  void *a = nullptr, *b = nullptr;
+  GCQ_HIP(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    gcqh::set_error("out of device memory for a %llu-byte copy", (unsigned long long)bytes);
+    return GCQ_ENOMEM;
+  }
+  hipStream_t s;
+  GCQ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const uint64_t n = bytes / 16;
+  const dim3 grid((uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)cus * 32)), blk(256);
+  GCQ_HIP(hipMemsetAsync(a, 1, bytes, s));
+  hipLaunchKernelGGL(k_copy, grid, blk, 0, s, (const uint4*)a, (uint4*)b, n);  // warm
+  hipEvent_t e0, e1;
+  GCQ_HIP(hipEventCreate(&e0));
+  GCQ_HIP(hipEventCreate(&e1));
+  GCQ_HIP(hipEventRecord(e0, s));
+  for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, grid, blk, 0, s, (const uint4*)a, (uint4*)b, n);
+  GCQ_HIP(hipEventRecord(e1, s));
+  GCQ_HIP(hipEventSynchronize(e1));
+  GCQ_HIP(hipGetLastError());
+  const double ms = gcqh::ms_between(e0, e1) / iters;
+  *gbps = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;  // read + write
+  if (kernel_ms) *kernel_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return GCQ_OK;
+}
+
 extern "C" int gcq_fence_cost(uint32_t threads, uint32_t block, uint32_t mode, uint32_t iters, double* ns) {
   GCQ_REQUIRE(ns && threads && block && iters && mode <= 3, "bad argument");
   uint64_t* d = nullptr;

@@ -99,6 +99,11 @@ int gcq_faa_peak(const gcq_faa_config* cfg, gcq_faa_result* res);
  * 1 store + release fence, 2 store + acquire fence, 3 store + seq_cst fence. */
 int gcq_fence_cost(uint32_t threads, uint32_t block, uint32_t mode, uint32_t iters, double* ns_per_iter);
 
+/* Streaming copy of `bytes` (read + write counted) averaged over iters
+ * launches: the measured HBM roofline, and a kernel of known traffic for
+ * calibrating HBM byte counters.  No reference counterpart. */
+int gcq_hbm_copy(uint64_t bytes, uint32_t iters, double* gbps, double* kernel_ms);
+
 /* ------------------------------------------------------------------ throughput */
 enum gcq_tb_mode { GCQ_TB_BALANCED = 0, GCQ_TB_SPLIT = 1 };
 typedef struct gcq_tb_config {
