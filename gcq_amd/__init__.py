@@ -105,6 +105,7 @@ def load():
         "gcq_faa_peak": (C.c_int, [_P(FaaConfig), _P(FaaResult)]),
         "gcq_fence_cost": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P(C.c_double)]),
         "gcq_hbm_copy": (C.c_int, [C.c_uint64, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
+        "gcq_dropin_selfcheck": (C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, _P(C.c_uint32)]),
         "gcq_tb_run": (C.c_int, [C.c_void_p, _P(TbConfig), _P(TbResult)]),
         "gcq_fifo_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _P(FifoResult)]),
         "gcq_hist_run": (C.c_int, [C.c_void_p, _P(HistConfig), C.c_void_p, C.c_size_t, _P(C.c_size_t)]),
@@ -249,3 +250,17 @@ def hbm_copy(nbytes=1 << 30, iters=20):
     gbps, ms = C.c_double(0), C.c_double(0)
     _check(load().gcq_hbm_copy(nbytes, iters, C.byref(gbps), C.byref(ms)), "gcq_hbm_copy")
     return gbps.value, ms.value
+
+
+DROPIN_APIS = ("gwf_ptr", "gwf_ring", "glf_ptr", "glf_ring", "wfq", "sfq_blocking", "sfq_nb")
+DROPIN_ERRORS = ("order", "capacity", "not_empty", "duplicate", "missing", "spin_limit")
+
+
+def dropin_selfcheck(api, threads=1024, ops=16):
+    """Run the drop-in header API `api` (index or name in DROPIN_APIS); returns
+    the error counters by name (all zero when the API behaves)."""
+    if isinstance(api, str):
+        api = DROPIN_APIS.index(api)
+    errs = (C.c_uint32 * 6)()
+    _check(load().gcq_dropin_selfcheck(api, threads, ops, errs), "gcq_dropin_selfcheck")
+    return dict(zip(DROPIN_ERRORS, list(errs)))

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libgcq.so")
 ARCH = os.environ.get("GCQ_OFFLOAD_ARCH", "gfx950")
 
 # Every HIP TU of the library.
-SOURCES = ["core.hip", "bfs.hip", "batch.hip", "graph.cpp", "rt.hip", "serve.hip"]
+SOURCES = ["core.hip", "bfs.hip", "batch.hip", "graph.cpp", "rt.hip", "serve.hip", "dropin_check.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -23,6 +23,7 @@ HIPCC_FLAGS = [
     "-Wall",
     "-Wno-unused-function",
     "-pthread",
+    f"-I{CSRC}",  # the drop-in headers in include/ reach the queue code as "gcq/..."
     # packed-FP32 forms are disabled for all device code (known-bad forms on
     # these boxes); hipcc's host pass warns that it does not know the feature.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -56,6 +56,8 @@ struct Geometry {
   uint32_t nthreads;    // handles
   uint32_t patience;
   uint32_t off_t, off_h, magic, pad;
+  uint64_t ext_cells;   // 0: cells follow the header; else their own allocation
+  uint64_t ext_handles; // handle array, for callers that pass only their own handle
 };
 constexpr uint32_t kMagic = 0x594D4331u;  // "YMC1"
 struct Queue {
@@ -65,7 +67,10 @@ struct Queue {
 static_assert(sizeof(Queue) == kHdrBytes, "header");
 
 __host__ __device__ constexpr uint64_t queue_bytes(uint64_t cells) { return kHdrBytes + 16ull * cells; }
-__device__ __forceinline__ Cell* cells(Queue* q) { return (Cell*)((char*)q + kHdrBytes); }
+__device__ __forceinline__ Cell* cells(Queue* q) {
+  const uint64_t ext = q->g.ext_cells;
+  return ext ? (Cell*)ext : (Cell*)((char*)q + kHdrBytes);
+}
 __device__ __forceinline__ uint64_t* Tp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_t); }
 __device__ __forceinline__ uint64_t* Hp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_h); }
 
@@ -106,6 +111,8 @@ __device__ inline void init_slice(Queue* q, Handle* h, uint64_t ncells, uint32_t
     g.off_h = 5 * kLine;
     g.magic = kMagic;
     g.pad = 0;
+    g.ext_cells = 0;
+    g.ext_handles = 0;
     q->g = g;
     *(uint64_t*)((char*)q + g.off_t) = 0;
     *(uint64_t*)((char*)q + g.off_h) = 0;
@@ -122,6 +129,7 @@ struct Ops {
   Probe* pr;
 
   __device__ __forceinline__ Cell* cell(uint64_t i) const { return i < ncells ? &cells(q)[i] : nullptr; }
+
   __device__ __forceinline__ uint32_t ref_of(uint32_t tid) const { return tid + 2u; }
   __device__ __forceinline__ Handle* handle_of(uint32_t ref) const { return &hs[ref - 2u]; }
 
