@@ -166,18 +166,26 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
     if (pr) { pr->step(); pr->after_ticket(); }
     const uint32_t tcyc = cycle_of(T, g.slots_log2);
     const uint32_t j = slot_of(T, g.slots_log2, g.remap_log2);
-    uint64_t E = ld_rlx(&e[j]);
-    for (;;) {
-      if (cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || ld_rlx(head(r)) <= T)) {
-        if (!cas_rlx(&e[j], E, e_pack(tcyc, true, idx))) {  // E refreshed
-          if (pr) pr->step();
-          continue;
-        }
-        if (ld_rlx(thr(r)) != g.thresh_max) xchg_rlx(thr(r), g.thresh_max);
-        return;
-      }
-      break;
+    // Speculative CAS: in steady state the slot holds the previous lap's
+    // consumed entry (cycle tcyc - 1, safe, index ⊥c), or the initial entry
+    // on the first lap.  When it does, one CAS replaces load + CAS;
+    // otherwise the CAS returns the current word and the usual checks run.
+    uint64_t E = (T >> g.slots_log2) == 1 ? kInit : e_pack(tcyc - 1u, true, kBotC);
+    bool ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));  // E refreshed on failure
+    while (!ok) {
+      if (pr) pr->step();
+      if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || ld_rlx(head(r)) <= T))) break;
+      ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));
     }
+    // Threshold back to 3n - 1 after the insertions (glf_queue.hpp:493-495):
+    // one store per wave, issued after every successful lane's CAS has
+    // returned, instead of a load (and exchange) per lane on the critical
+    // path.  Storing 3n - 1 when it already holds 3n - 1 is harmless, and a
+    // reset that lands after a concurrent decrement is the same as a late
+    // reset.
+    const uint64_t okm = __ballot(ok);
+    if (okm && lane_id() == first_lane(okm)) st_rlx(thr(r), g.thresh_max);
+    if (ok) return;
   }
 }
 

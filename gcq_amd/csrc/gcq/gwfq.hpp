@@ -253,19 +253,28 @@ struct RingOps {
     pr->after_ticket();
     const uint32_t tcyc = cyc(T), j = slot(T);
     uint64_t* const e = slots(r);
-    uint64_t E = ld_rlx(&e[j]);
-    for (;;) {
-      if (cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot &&
-          (e_safe(E) || cnt_of(ld_rlx(head(r))) <= T)) {
-        if (!cas_rlx(&e[j], E, e_pack(e_note(E), tcyc, true, true, 0, idx))) {
-          pr->step();
-          continue;
-        }
-        if (ld_rlx(thr(r)) != g.thresh_max) xchg_rlx(thr(r), g.thresh_max);
-        return 0;
-      }
-      return T;
+    // Speculative CAS: in steady state the slot holds the previous lap's
+    // consumed entry (fast-path note, cycle tcyc - 1, safe, Enq, index ⊥c),
+    // or the initial entry on the first lap.  When it does, one CAS replaces
+    // load + CAS; otherwise the CAS returns the current word and the usual
+    // checks run on it.
+    uint64_t E = (T >> g.slots_log2) == 1 ? kInit : e_pack(0xFF, tcyc - 1u, true, true, 0, kBotC);
+    bool ok = cas_rlx(&e[j], E, e_pack(0xFF, tcyc, true, true, 0, idx));
+    while (!ok) {
+      pr->step();
+      if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || cnt_of(ld_rlx(head(r))) <= T)))
+        break;
+      ok = cas_rlx(&e[j], E, e_pack(e_note(E), tcyc, true, true, 0, idx));
     }
+    // Threshold back to 3n - 1 after the insertions (gwf_ring.hpp:409-410):
+    // one store per wave, issued after every successful lane's CAS has
+    // returned, instead of a load (and exchange) per lane on the critical
+    // path.  Storing 3n - 1 when it already holds 3n - 1 is harmless, and a
+    // reset that lands after a concurrent decrement is the same as a late
+    // reset.
+    const uint64_t okm = __ballot(ok);
+    if (okm && lane_id() == first_lane(okm)) st_rlx(thr(r), g.thresh_max);
+    return ok ? 0 : T;
   }
 
   // Fast path dequeue attempt.  0 = done (*out = index or kBot), else ticket.

@@ -238,13 +238,50 @@ struct Ops {
     return ld_rlx(&c->val);
   }
 
-  __device__ uint64_t deq_fast(uint64_t* id) {  // :416-434
-    const uint64_t i = wave_faa(Hp(q), (uint64_t)1);
+  // Head ticket bounded by the cells enqueuers have taken.  The wave leader
+  // reads H, then T, and takes k = min(active lanes, T - H) tickets with one
+  // FAA; lanes ranked below k get base + rank, the others claim again.  T <=
+  // H at the leader's T read is EMPTY for every lane still claiming (the
+  // early-empty rule of dequeue()).  Without the bound, a wave of 64 idle
+  // consumers takes 64 cells for one pending item and marks 63 of them TOP,
+  // and every enqueuer that later draws one of those cells has to retry.
+  __device__ __forceinline__ bool claim_h(uint64_t* ticket) {
+    for (;;) {
+      const uint64_t m = active_mask();
+      const uint32_t leader = first_lane(m);
+      const uint32_t n = (uint32_t)__builtin_popcountll(m);
+      uint64_t base = 0;
+      uint32_t k = 0;
+      if (lane_id() == leader) {
+        const uint64_t h0 = ld_rlx(Hp(q));
+        vm_wait();  // H is read before T
+        const uint64_t t0 = ld_rlx(Tp(q));
+        const uint64_t avail = t0 > h0 ? t0 - h0 : 0;
+        k = avail < n ? (uint32_t)avail : n;
+        if (k) base = faa_rlx(Hp(q), (uint64_t)k);
+      }
+      k = bcast32(k, leader);
+      if (k == 0) return false;
+      base = bcast64(base, leader);
+      const uint32_t rank = rank_in(m);
+      if (rank < k) {
+        *ticket = base + rank;
+        return true;
+      }
+    }
+  }
+
+  __device__ uint64_t deq_fast(uint64_t i) {  // :416-434, ticket from claim_h
     pr->step();
     pr->after_ticket();
-    *id = i;
     Cell* c = cell(i);
     if (!c) return kEmpty;
+    // The ticket lies below T: its enqueuer is in flight.  Give it a moment
+    // before help_enq marks the cell TOP and sends the enqueuer round again.
+    for (uint32_t w = 0; w < kSlotWaitSpins && ld_rlx(&c->val) == kBottom; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      pr->step();
+    }
     const uint64_t v = help_enq(c, i);
     if (v == kEmpty) return kEmpty;
     if (v != kTop && v != kRetry) {
@@ -318,17 +355,13 @@ struct Ops {
   }
 
   __device__ uint64_t dequeue() {  // :538-555
-    // Empty test that takes no cell: H is read, then T; T <= H means every
-    // enqueue cell is already owned by a dequeuer (EMPTY at the T read).
-    // Idle consumers would otherwise burn a cell per attempt.
-    {
-      const uint64_t h0 = ld_rlx(Hp(q));
-      vm_wait();
-      if (ld_rlx(Tp(q)) <= h0) return kEmpty;
-    }
+    // Every ticket comes from claim_h, which takes no cell when T <= H: then
+    // every enqueue cell is already owned by a dequeuer (EMPTY at the T
+    // read).  Idle consumers would otherwise burn a cell per attempt.
     uint64_t v = kRetry, cid = 0;
     for (int p = (int)q->g.patience; p >= 0; --p) {
-      v = deq_fast(&cid);
+      if (!claim_h(&cid)) return kEmpty;
+      v = deq_fast(cid);
       if (v != kRetry) break;
     }
     if (v == kRetry) v = deq_slow(cid);
