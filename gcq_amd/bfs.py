@@ -2,6 +2,7 @@
 concurrent_bfs harness, bfs_bench.cpp): synthetic graphs, single BFS runs
 and the batched/served paths, all through libgcq's C ABI."""
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -27,6 +28,16 @@ class Graph:
     def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=11):
         h = C.c_void_p()
         _check(load().gcq_graph_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)), "gcq_graph_rmat")
+        return cls(h)
+
+    @classmethod
+    def load(cls, path, fmt="auto", undirected=True):
+        """A Matrix Market (.mtx) or SNAP edge-list file, read with the
+        reference loaders' rules (bfs_bench.cpp:202-243); fmt "auto" picks by
+        the ".mtx" suffix."""
+        code = {"auto": 0, "mtx": 1, "snap": 2}[fmt]
+        h = C.c_void_p()
+        _check(load().gcq_graph_load(os.fsencode(path), code, int(bool(undirected)), C.byref(h)), "gcq_graph_load")
         return cls(h)
 
     @classmethod

@@ -317,6 +317,18 @@ extern "C" int gcq_graph_rmat(int32_t scale, int32_t edge_factor, double a, doub
   return finish_create(g, out);
 }
 
+extern "C" int gcq_graph_load(const char* path, int32_t format, int32_t undirected, gcq_graph** out) {
+  GCQ_REQUIRE(out && path, "null argument");
+  GCQ_REQUIRE(format >= GCQ_GRAPH_AUTO && format <= GCQ_GRAPH_SNAP, "format must be GCQ_GRAPH_AUTO, _MTX or _SNAP");
+  gcq_graph* g = new gcq_graph();
+  std::string err;
+  if (gcqg::load_edge_file(path, format, undirected != 0, &g->host, &err) != 0) {
+    delete g;
+    GCQ_REQUIRE(false, "%s", err.c_str());
+  }
+  return finish_create(g, out);
+}
+
 extern "C" int gcq_graph_from_csr(int32_t nv, int64_t ne, const int32_t* offsets, const int32_t* indices,
                                   gcq_graph** out) {
   GCQ_REQUIRE(out && offsets && (indices || ne == 0) && nv > 0 && ne >= 0 && ne < (1ll << 31), "bad CSR");

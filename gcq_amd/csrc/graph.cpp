@@ -14,9 +14,15 @@
 //   2^scale generated edges, each endpoint pair drawn by recursive quadrant
 //   choice with probabilities (a, b, c, 1-a-b-c); symmetrised, self loops and
 //   duplicates removed (radix sort of (u,v) keys).
+// load_edge_file(path, format, undirected): Matrix Market (.mtx) and SNAP
+//   edge-list files with the reference loaders' rules (load_mtx / load_snap,
+//   bfs_bench.cpp:202-243), parsed from one read of the file and assembled
+//   by a counting sort on the source vertex plus a per-row sort and dedup.
 #include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -154,6 +160,176 @@ int32_t pick_source(const Csr& g, uint64_t seed, uint64_t k) {
   for (uint64_t t = 0; t < 1000000; ++t) {
     const int32_t v = (int32_t)(draw(seed, 97, k * 1000003ull + t) % (uint64_t)g.nv);
     if (g.offsets[v + 1] > g.offsets[v]) return v;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ edge-list files
+namespace {
+
+// One line of text: [b, e).
+struct Line {
+  const char* b;
+  const char* e;
+};
+
+// Parse a signed decimal integer as `istream >> long long` does: skip
+// leading blanks, optional sign, at least one digit, stop at the first
+// non-digit.  False on no digits or overflow.
+bool parse_ll(const char*& p, const char* e, long long* out) {
+  while (p < e && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
+  bool neg = false;
+  if (p < e && (*p == '+' || *p == '-')) neg = *p++ == '-';
+  if (p >= e || *p < '0' || *p > '9') return false;
+  unsigned long long v = 0;
+  while (p < e && *p >= '0' && *p <= '9') {
+    const unsigned d = (unsigned)(*p++ - '0');
+    if (v > (9223372036854775807ull - d) / 10ull) return false;  // out of range for long long
+    v = v * 10ull + d;
+  }
+  *out = neg ? -(long long)v : (long long)v;
+  return true;
+}
+
+bool read_file(const char* path, std::string* buf) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return false;
+  char tmp[1 << 16];
+  size_t n;
+  while ((n = std::fread(tmp, 1, sizeof tmp, f)) > 0) buf->append(tmp, n);
+  const bool ok = !std::ferror(f);
+  std::fclose(f);
+  return ok;
+}
+
+template <class F>
+void for_lines(const std::string& buf, F f) {
+  const char* p = buf.data();
+  const char* end = p + buf.size();
+  while (p < end) {
+    const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
+    const char* le = nl ? nl : end;
+    if (!f(Line{p, le})) return;
+    p = nl ? nl + 1 : end;
+  }
+}
+
+// Sorted, deduplicated CSR over V vertices from (u, v) pairs; pairs with an
+// endpoint outside [0, V) are dropped (csr_from_edges, bfs_bench.cpp:185-200).
+void csr_from_pairs(int64_t V, const std::vector<int64_t>& us, const std::vector<int64_t>& vs, Csr* g) {
+  g->nv = (int32_t)V;
+  std::vector<int64_t> cnt(V + 1, 0);
+  const size_t m = us.size();
+  for (size_t i = 0; i < m; ++i)
+    if (us[i] >= 0 && us[i] < V && vs[i] >= 0 && vs[i] < V) ++cnt[us[i] + 1];
+  for (int64_t v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
+  std::vector<int32_t> adj(cnt[V]);
+  std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+  for (size_t i = 0; i < m; ++i)
+    if (us[i] >= 0 && us[i] < V && vs[i] >= 0 && vs[i] < V) adj[cur[us[i]]++] = (int32_t)vs[i];
+  g->offsets.assign(V + 1, 0);
+  g->indices.clear();
+  g->indices.reserve(adj.size());
+  for (int64_t v = 0; v < V; ++v) {
+    int32_t* b = adj.data() + cnt[v];
+    int32_t* e = adj.data() + cnt[v + 1];
+    std::sort(b, e);
+    e = std::unique(b, e);
+    g->indices.insert(g->indices.end(), b, e);
+    g->offsets[v + 1] = (int32_t)g->indices.size();
+  }
+}
+
+}  // namespace
+
+int load_edge_file(const char* path, int format, bool undirected, Csr* g, std::string* err) {
+  std::string buf;
+  if (!read_file(path, &buf)) {
+    *err = std::string("cannot read ") + path;
+    return -1;
+  }
+  if (format == kFormatAuto) {
+    const size_t n = std::strlen(path);
+    format = (n >= 4 && std::strcmp(path + n - 4, ".mtx") == 0) ? kFormatMtx : kFormatSnap;
+  }
+  std::vector<int64_t> us, vs;
+  auto add = [&](int64_t u, int64_t v) {
+    us.push_back(u);
+    vs.push_back(v);
+    if (undirected) {
+      us.push_back(v);
+      vs.push_back(u);
+    }
+  };
+  int64_t V = 0;
+  if (format == kFormatMtx) {
+    // Banner, comment lines, "rows cols nnz", then nnz "i j [value]" lines
+    // (1-based); self loops and out-of-range entries are counted but not
+    // kept; V = max(rows, cols).
+    if (buf.compare(0, 14, "%%MatrixMarket") != 0) {
+      *err = std::string(path) + ": no %%MatrixMarket banner";
+      return -1;
+    }
+    long long nr = -1, nc = -1, nz = -1, cnt = 0;
+    bool first = true, sized = false, bad_size = false;
+    for_lines(buf, [&](Line l) {
+      if (first) {
+        first = false;
+        return true;  // the banner
+      }
+      if (l.b == l.e || *l.b == '%') return true;
+      const char* p = l.b;
+      if (!sized) {
+        sized = true;
+        bad_size = !(parse_ll(p, l.e, &nr) && parse_ll(p, l.e, &nc) && parse_ll(p, l.e, &nz)) || nr < 0 || nc < 0 ||
+                   nz < 0 || nr > 2147483647ll || nc > 2147483647ll;
+        return !bad_size && nz > 0;
+      }
+      long long i, j;
+      if (!(parse_ll(p, l.e, &i) && parse_ll(p, l.e, &j))) return true;
+      --i;
+      --j;
+      ++cnt;
+      if (i != j && i >= 0 && j >= 0 && i < nr && j < nc) add(i, j);
+      return cnt < nz;
+    });
+    if (!sized || bad_size) {
+      *err = std::string(path) + ": bad or missing size line";
+      return -1;
+    }
+    V = std::max(nr, nc);
+  } else if (format == kFormatSnap) {
+    // "u v" lines, '#' / '%' comments; self loops dropped; ids shifted down
+    // by one when the smallest id is 1; V = largest id + 1.
+    long long mn = 1ll << 60, mx = -1;
+    for_lines(buf, [&](Line l) {
+      if (l.b == l.e || *l.b == '#' || *l.b == '%') return true;
+      const char* p = l.b;
+      long long u, v;
+      if (!(parse_ll(p, l.e, &u) && parse_ll(p, l.e, &v)) || u == v) return true;
+      mn = std::min(mn, std::min(u, v));
+      mx = std::max(mx, std::max(u, v));
+      add(u, v);
+      return true;
+    });
+    if (mn == 1) {
+      for (auto& x : us) --x;
+      for (auto& x : vs) --x;
+      --mx;
+    }
+    V = mx + 1;
+  } else {
+    *err = "unknown graph file format";
+    return -1;
+  }
+  if (V <= 0 || V > 2147483646ll) {
+    *err = std::string(path) + ": no vertices (or more than 2^31 - 2)";
+    return -1;
+  }
+  csr_from_pairs(V, us, vs, g);
+  if ((int64_t)g->indices.size() >= (1ll << 31)) {
+    *err = std::string(path) + ": more than 2^31 - 1 edges";
+    return -1;
   }
   return 0;
 }

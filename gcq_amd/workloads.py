@@ -75,6 +75,7 @@ SIGNATURES = {
     "gcq_graph_lattice": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, _P(VP)]),
     "gcq_graph_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, _P(VP)]),
     "gcq_graph_from_csr": (C.c_int, [C.c_int32, C.c_int64, VP, VP, _P(VP)]),
+    "gcq_graph_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _P(VP)]),
     "gcq_graph_info": (C.c_int, [VP, _P(C.c_int32), _P(C.c_int64)]),
     "gcq_graph_copy_csr": (C.c_int, [VP, VP, VP]),
     "gcq_graph_pick_source": (C.c_int32, [VP, C.c_uint64, C.c_uint64]),

@@ -177,6 +177,18 @@ int gcq_graph_lattice(int32_t rows, int32_t cols, double keep, double diag, uint
  * symmetrised and deduplicated.  No reference counterpart. */
 int gcq_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
                    gcq_graph** out);
+/* Load a graph file with the reference loaders' rules (load_mtx / load_snap,
+ * bfs_bench.cpp:202-243; format chosen by the ".mtx" suffix under
+ * GCQ_GRAPH_AUTO as at bfs_bench.cpp:691-692).  Matrix Market: banner,
+ * "rows cols nnz", then nnz 1-based "i j [value]" entries; self loops and
+ * out-of-range entries are skipped; V = max(rows, cols).  SNAP: "u v" lines
+ * with '#'/'%' comments; self loops skipped; ids shifted down by one when
+ * the smallest id is 1; V = largest id + 1.  `undirected` stores both
+ * directions.  The CSR is sorted and deduplicated.  Unlike the reference, a
+ * missing banner or size line, an empty graph, or more than 2^31 - 2
+ * vertices is an error (GCQ_EINVAL) rather than an empty graph. */
+enum gcq_graph_format { GCQ_GRAPH_AUTO = 0, GCQ_GRAPH_MTX = 1, GCQ_GRAPH_SNAP = 2 };
+int gcq_graph_load(const char* path, int32_t format, int32_t undirected, gcq_graph** out);
 /* Upload a caller CSR (sorted, symmetric) to HBM; replaces the CSR H2D copy
  * of bfs_bench.cpp:703-708. */
 int gcq_graph_from_csr(int32_t nv, int64_t ne, const int32_t* offsets, const int32_t* indices, gcq_graph** out);

@@ -115,3 +115,21 @@ def test_kind_switch_same_graph(gcq, orc):
         src = g.pick_source(3, 0)
         for kind in (0, 3, 1, 2, 0):
             _run_and_check(orc, g, src, kind, 2048, 1, off, idx)
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_graph_from_file(gcq, orc, kind, tmp_path):
+    # A graph read from a SNAP edge list (bfs_bench.cpp:226-243 rules) runs
+    # like any other CSR: lattice 120 x 150, one direction per line, 1-based.
+    from gcq_amd import bfs
+    off, idx = bfs.lattice(120, 150, keep=0.75, diag=0.01, seed=3)
+    nv = len(off) - 1
+    src = np.repeat(np.arange(nv), np.diff(off))
+    up = src < idx
+    path = tmp_path / "lattice.txt"
+    path.write_text("# lattice\n" + "".join(f"{u + 1} {v + 1}\n" for u, v in zip(src[up], idx[up])))
+    with bfs.Graph.load(str(path)) as g:
+        o2, i2 = g.csr()
+        if np.diff(off)[0] > 0 and np.diff(off)[-1] > 0:  # ids 1..nv all present: the same CSR
+            assert np.array_equal(o2, off) and np.array_equal(i2, idx)
+        _run_and_check(orc, g, 0, kind, 2048, 1, o2, i2)
