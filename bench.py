@@ -28,7 +28,8 @@ metric, each measured in this run ("secondary"):
     ray-traced tiles of the 64-sphere scene, 64 GB budget) through the
     continuously batched server and through plain per-request launches:
     p50 / p99 latency and throughput, plus both capacities on a burst;
-  * rt: the 1920 x 1080 64-sphere frame, 4 bounces, 256 x 256 tiles: Mrays/s.
+  * rt: the 1920 x 1080 64-sphere frame, 4 bounces, 256 x 256 tiles: Mrays/s,
+    beside the stream-compaction renderer on the same frame.
 """
 import argparse
 import json
@@ -251,6 +252,16 @@ def secondary(g, with_cpu):
     out["rt"] = {"workload": "1920x1080, 64 random spheres + ground, seed 3, 4 bounces, 256x256 tiles, G-LFQ",
                  "rays": res["total_rays"], "kernel_ms_median": round(med, 3),
                  "mrays_per_s": round(res["total_rays"] / med / 1e3, 1)}
+    # the stream-compaction comparator on the same frame (same image)
+    scene.render_compact(width=1920, height=1080)
+    cms = []
+    for _ in range(5):
+        _, cres = scene.render_compact(width=1920, height=1080)
+        cms.append(cres["kernel_ms"])
+    cmed = float(np.median(cms))
+    out["rt"]["compaction"] = {"frame_ms_median": round(cmed, 3), "rays": cres["total_rays"],
+                               "mrays_per_s": round(cres["total_rays"] / cmed / 1e3, 1),
+                               "queue_frame_over_compaction_frame": round(med / cmed, 3)}
     scene.close()
     return out
 

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libgcq.so")
 ARCH = os.environ.get("GCQ_OFFLOAD_ARCH", "gfx950")
 
 # Every HIP TU of the library.
-SOURCES = ["core.hip", "bfs.hip", "batch.hip", "graph.cpp", "rt.hip", "serve.hip", "dropin_check.hip"]
+SOURCES = ["core.hip", "bfs.hip", "batch.hip", "graph.cpp", "rt.hip", "rt_compact.hip", "serve.hip", "dropin_check.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -430,6 +430,7 @@ extern "C" int gcq_rt_scene_destroy(gcq_rt_scene* s) {
   if (!s) return GCQ_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_ctx(s);
+  rtk::free_compact(s);
   if (s->d_sph) (void)hipFree(s->d_sph);
   if (s->d_pl) (void)hipFree(s->d_pl);
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -27,6 +27,14 @@ struct gcq_rt_scene {
   uint64_t* counters = nullptr;  // alive, traced per tile
   uint32_t counters_cap = 0;
   uint32_t* err = nullptr;
+  // stream-compaction renderer (rt_compact.hip)
+  rtk::RayRec* cbuf[2] = {nullptr, nullptr};  // this bounce's rays, the next bounce's
+  uint64_t cbuf_cap = 0;                      // records per buffer
+  uint64_t* cstatus = nullptr;                // per-block scan status
+  uint32_t cstatus_cap = 0;
+  uint32_t* cctl = nullptr;                   // block-id ticket, next-bounce count
+  float* cimg = nullptr;
+  uint64_t cimg_cap = 0;
 };
 
 namespace rtk {
@@ -35,4 +43,6 @@ int scene_upload(gcq_rt_scene* s);
 // The kernel-side view of an uploaded scene (camera normalised on the host,
 // rt_final.cpp:851-859).
 Scene scene_view(const gcq_rt_scene* s);
+// Release the stream-compaction renderer's buffers.
+void free_compact(gcq_rt_scene* s);
 }  // namespace rtk

@@ -73,6 +73,19 @@ class Scene:
         _check(load().gcq_rt_render(self.handle, C.byref(cfg), img.ctypes.data, C.byref(res)), "gcq_rt_render")
         return img, {f: getattr(res, f) for f, _ in RtResult._fields_ if f != "pad"}
 
+    def render_compact(self, cfg=None, **kw):
+        """The stream-compaction comparator (one launch per bounce, live rays
+        packed by a prefix sum); same image as render().  Returns
+        (image[h, w, 3] float32, result dict)."""
+        cfg = cfg if cfg is not None else config(**kw)
+        w = cfg.w or cfg.width
+        h = cfg.h or cfg.height
+        img = np.empty((h, w, 3), np.float32)
+        res = RtResult()
+        _check(load().gcq_rt_render_compact(self.handle, C.byref(cfg), img.ctypes.data, C.byref(res)),
+               "gcq_rt_render_compact")
+        return img, {f: getattr(res, f) for f, _ in RtResult._fields_ if f != "pad"}
+
     def close(self):
         if self._h is not None:
             load().gcq_rt_scene_destroy(self._h)

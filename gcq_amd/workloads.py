@@ -93,6 +93,7 @@ SIGNATURES = {
     "gcq_rt_scene_destroy": (C.c_int, [VP]),
     "gcq_rt_config_default": (None, [_P(RtConfig)]),
     "gcq_rt_render": (C.c_int, [VP, _P(RtConfig), VP, _P(RtResult)]),
+    "gcq_rt_render_compact": (C.c_int, [VP, _P(RtConfig), VP, _P(RtResult)]),
     "gcq_server_config_default": (None, [_P(ServerConfig)]),
     "gcq_server_create": (C.c_int, [_P(VP), C.c_int32, VP, C.c_int32, C.c_int32, _P(ServerConfig), _P(VP)]),
     "gcq_server_start": (C.c_int, [VP]),

@@ -324,6 +324,14 @@ typedef struct gcq_rt_result {
 void gcq_rt_config_default(gcq_rt_config* cfg);
 /* Render cfg's region into img_out (w * h RGB float triples, row-major). */
 int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* img_out, gcq_rt_result* res);
+/* The stream-compaction comparator (concurrent_rt/baselines/rt_compaction.cpp:
+ * 219-472, 748-833): one launch per bounce traces every live ray and packs
+ * the continuing ones for the next bounce with a prefix sum, with the same
+ * shading as gcq_rt_render, so the image is the same.  Uses cfg's image,
+ * region, max_depth and block; kind, threads_per_tile and the tiling are
+ * ignored.  kernel_ms is the frame: first launch to last kernel end,
+ * including the host's read of each bounce's ray count. */
+int gcq_rt_render_compact(gcq_rt_scene* s, const gcq_rt_config* cfg, float* img_out, gcq_rt_result* res);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference.  A continuously batched server: BFS

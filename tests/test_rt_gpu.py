@@ -71,3 +71,30 @@ def test_threads_and_blocks(gcq, orc):
         for tpt, blk in ((64, 64), (512, 128), (4096, 512)):
             img, res = s.render(width=640, height=360, threads_per_tile=tpt, block=blk)
             _check(orc, s, img, res, 640, 360)
+
+
+@pytest.mark.parametrize("preset", [0, 1, 2], ids=["complex", "cornell", "random64"])
+def test_compaction_renderer_same_image(gcq, orc, preset):
+    # The stream-compaction comparator (one launch per bounce, prefix-sum
+    # packing) shades exactly as the queue renderer: identical images.
+    from gcq_amd import rt
+    W, H = (1920, 1080) if preset == 2 else (1280, 720)
+    with (rt.Scene.preset(rt.RANDOM, 64, 3) if preset == 2 else rt.Scene.preset(preset)) as s:
+        qimg, qres = s.render(width=W, height=H, tile_w=256, tile_h=256)
+        for blk in (256, 64, 1024):
+            print(f"[rt compact] preset={preset} block={blk}", flush=True)
+            img, res = s.render_compact(width=W, height=H, block=blk)
+            assert np.array_equal(img, qimg)
+            assert res["total_rays"] == qres["total_rays"]
+        _check(orc, s, img, res, W, H)
+
+
+@pytest.mark.parametrize("max_depth", [0, 1, 8])
+def test_compaction_renderer_regions_and_depths(gcq, orc, max_depth):
+    from gcq_amd import rt
+    with rt.Scene.preset(rt.CORNELL) as s:
+        img, res = s.render_compact(width=480, height=270, max_depth=max_depth)
+        _check(orc, s, img, res, 480, 270, max_depth=max_depth)
+        sub, sres = s.render_compact(width=480, height=270, x0=100, y0=30, w=77, h=101, max_depth=max_depth)
+        assert np.array_equal(sub, img[30:131, 100:177])
+        _check(orc, s, sub, sres, 480, 270, 100, 30, 77, 101, max_depth=max_depth)
