@@ -223,13 +223,14 @@ def secondary(g, with_cpu):
     lam = 150.0
     rows = serve.poisson_trace(300, lam, nvs, frame, rt_fraction=0.1, seed=5, graph_sources=pick)
     burst = serve.poisson_trace(300, 1e9, nvs, frame, rt_fraction=0.1, seed=6, graph_sources=pick)
-    scfg = dict(bfs_threads=196608, block=512, rt_threads=16384)
+    scfg = dict(bfs_threads=229376, block=512, rt_threads=32768, frontier_log2=28)
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         s_trace = serve.latency_summary(srv.run_trace(rows))
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         s_burst = serve.latency_summary(srv.run_trace(burst))
-    p_trace = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows, bfs_threads=32768))
-    p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, bfs_threads=32768))
+    pcfg = dict(bfs_threads=65536, rt_threads_per_tile=8192)
+    p_trace = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows, **pcfg))
+    p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, **pcfg))
     rnd = lambda d: {k: round(v, 3) if isinstance(v, float) else v for k, v in d.items()}  # noqa: E731
     out["served"] = {
         "workload": f"Poisson {lam:.0f} req/s, 300 requests: 90% BFS on R-MAT scales 14..22, 10% 256x256 tiles "
@@ -237,26 +238,29 @@ def secondary(g, with_cpu):
         "served": rnd(s_trace), "plain": rnd(p_trace),
         "burst_served": rnd(s_burst), "burst_plain": rnd(p_burst),
         "capacity_vs_plain": round(s_burst["throughput_rps"] / p_burst["throughput_rps"], 2),
-        "config": dict(scfg, plain_bfs_threads=32768, plain_rt_threads_per_tile=1024)}
+        "config": dict(scfg, plain_bfs_threads=pcfg["bfs_threads"],
+                       plain_rt_threads_per_tile=pcfg["rt_threads_per_tile"])}
     for gr in graphs:
         gr.close()
     _progress(f"served p99 {s_trace['p99_ms']:.1f} ms vs plain {p_trace['p99_ms']:.1f} ms")
 
     # --- ray tracing
-    scene.render(width=1920, height=1080, tile_w=256, tile_h=256)
+    rcfg = dict(width=1920, height=1080, tile_w=256, tile_h=256, threads_per_tile=8192, block=512)
+    scene.render(**rcfg)
     ms = []
     for _ in range(5):
-        _, res = scene.render(width=1920, height=1080, tile_w=256, tile_h=256)
+        _, res = scene.render(**rcfg)
         ms.append(res["kernel_ms"])
     med = float(np.median(ms))
     out["rt"] = {"workload": "1920x1080, 64 random spheres + ground, seed 3, 4 bounces, 256x256 tiles, G-LFQ",
                  "rays": res["total_rays"], "kernel_ms_median": round(med, 3),
-                 "mrays_per_s": round(res["total_rays"] / med / 1e3, 1)}
+                 "mrays_per_s": round(res["total_rays"] / med / 1e3, 1),
+                 "config": {"threads_per_tile": 8192, "block": 512}}
     # the stream-compaction comparator on the same frame (same image)
-    scene.render_compact(width=1920, height=1080)
+    scene.render_compact(width=1920, height=1080, block=1024)
     cms = []
     for _ in range(5):
-        _, cres = scene.render_compact(width=1920, height=1080)
+        _, cres = scene.render_compact(width=1920, height=1080, block=1024)
         cms.append(cres["kernel_ms"])
     cmed = float(np.median(cms))
     out["rt"]["compaction"] = {"frame_ms_median": round(cmed, 3), "rays": cres["total_rays"],
