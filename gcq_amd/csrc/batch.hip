@@ -47,7 +47,7 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
-    const uint32_t dw = wave_sum32(expand(fq.in(L), fq.out(L), d, L, tid, st));
+    const uint32_t dw = wave_sum32(expand<F, false>(fq.in(L), fq.out(L), d, L, tid, st));
     if (lane_id() == 0 && dw) gcq::faa_rlx(&ctl->next[L % 3], dw);
     if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();

@@ -93,7 +93,9 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
       uint32_t w = 0;
       if (e < total) {
         w = (uint32_t)idx[obeg + (ec - oexcl)];
-        if (gcq::ld_rlx(&level[w]) == -1) {
+        // A depth changes once per traversal, from -1 to a level, so a cached
+        // (possibly stale) read can only send a visited vertex to the CAS.
+        if (gcq::ld_cached(&level[w]) == -1) {
           int32_t expect = -1;
           disc = gcq::cas_rlx(&level[w], expect, L + 1);
         }

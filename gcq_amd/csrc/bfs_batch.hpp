@@ -39,9 +39,14 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int lane) {
 }
 
 // Drain the level's input queue; returns the lane's discoveries.  All 64
-// lanes of the wave call it together.
-template <class F>
+// lanes of the wave call it together.  kLive: slots and depth arrays may be
+// rewritten while the kernel runs (the server admits queries into freed
+// slots), so they are read at device scope; otherwise (one batch per launch)
+// slots are fixed and a depth only goes from -1 to a level, so the visited
+// filter and the slot fields are read through the caches.
+template <class F, bool kLive = true>
 __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L, uint32_t tid, uint32_t* stage) {
+  auto rd = [](const auto* p) { return kLive ? gcq::ld_rlx(p) : gcq::ld_cached(p); };
   const uint32_t lane = lane_id();
   const uint32_t vmask = (1u << d.vbits) - 1u;
   uint32_t sc = 0, disc_n = 0;
@@ -57,10 +62,10 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
       slot = item >> d.vbits;
       const uint32_t v = item & vmask;
       const Slot* s = &d.slots[slot];
-      const int32_t* off = (const int32_t*)gcq::ld_rlx(&s->off);
-      idxp = gcq::ld_rlx(&s->idx);
-      depp = gcq::ld_rlx(&s->depth);
-      dn = L - (int32_t)gcq::ld_rlx(&s->start) + 1;
+      const int32_t* off = (const int32_t*)rd(&s->off);
+      idxp = rd(&s->idx);
+      depp = rd(&s->depth);
+      dn = L - (int32_t)rd(&s->start) + 1;
       beg = off[v];  // CSR arrays are uploaded before launch and never written
       deg = off[v + 1] - beg;
     }
@@ -91,7 +96,7 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
       if (e < total) {
         w = (uint32_t)((const int32_t*)oidx)[obeg + (ec - oexcl)];
         int32_t* dp = (int32_t*)odep + w;
-        if (gcq::ld_rlx(dp) == -1) {
+        if (rd(dp) == -1) {
           int32_t expect = -1;
           disc = gcq::cas_rlx(dp, expect, odn);
         }

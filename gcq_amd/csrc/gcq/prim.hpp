@@ -32,6 +32,15 @@ template <class T>
 __device__ __forceinline__ T ld_rlx(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Relaxed load that may be served by this CU's or XCD's caches (workgroup
+// scope): for words that change at most once during a kernel and only one
+// way (a depth going from -1 to a level), where a stale value costs no more
+// than a redundant atomic, and for words that do not change at all during
+// the kernel.
+template <class T>
+__device__ __forceinline__ T ld_cached(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 // System scope: words in host-pinned memory shared with host threads.
 template <class T>
 __device__ __forceinline__ T ld_sys(const T* p) {

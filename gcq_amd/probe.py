@@ -14,7 +14,7 @@ import sys
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="gcq_amd.probe")
-    ap.add_argument("what", choices=["copy", "tb", "bfs", "batch", "rt"])
+    ap.add_argument("what", choices=["copy", "tb", "bfs", "batch", "rt", "rtc"])
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--launches", type=int, default=20)
@@ -28,7 +28,7 @@ def main(argv=None):
         out.update(bytes=a.bytes, gbps=gbps, kernel_ms=ms)
     elif a.what == "tb":
         # the headline configuration: split 25/75, 65,536 threads, 128 ops, capacity 2^20
-        with g.Queue(g.KINDS[a.queue], capacity=1 << 20, max_threads=65536) as q:
+        with g.Queue(g.KINDS[a.queue], capacity=1 << 20, max_threads=65536, poll_backoff=0) as q:
             r = q.tb_run(g.SPLIT, threads=65536, block=256, chunk_ops=128, producer_percent=25,
                          warmup_launches=5, timed_launches=a.launches)
         out.update(r)
@@ -47,11 +47,14 @@ def main(argv=None):
         out.update(res)
         for gr in graphs:
             gr.close()
-    elif a.what == "rt":
+    elif a.what in ("rt", "rtc"):
         from gcq_amd import rt
         with rt.Scene.preset(rt.RANDOM, 64, 3) as s:
             for _ in range(a.runs):
-                _, res = s.render(width=1920, height=1080, tile_w=256, tile_h=256)
+                if a.what == "rt":
+                    _, res = s.render(width=1920, height=1080, tile_w=256, tile_h=256, threads_per_tile=8192, block=512)
+                else:
+                    _, res = s.render_compact(width=1920, height=1080, block=1024)
         out.update(res)
     print(json.dumps(out), flush=True)
 

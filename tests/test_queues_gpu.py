@@ -15,6 +15,12 @@ def _queue(gcq, kind, threads, capacity=32768, **kw):
     return gcq.Queue(kind, capacity=capacity, max_threads=max(threads, 64), **kw)
 
 
+# Largest step count (shared-memory accesses of one operation) accepted under
+# the delay adversary below.  Measured on MI355X: G-WFQ p50/p99/max 2/102/919
+# steps with 1,839 slow-path operations; WFQ 1/50/211 with 1,344.
+STEP_BOUND = {0: 20_000, 2: 5_000}
+
+
 def _check(orc, h, what):
     rc, msg = orc.lincheck(h, orc.AUTO, 20.0)
     assert rc == orc.OK, f"{what}: {msg}"
@@ -90,3 +96,25 @@ def test_gwfq_slow_path_history(gcq, orc):
         h = q.hist_run(threads=256, ops=16, block=64, mode=0)
     assert h["slow"].sum() > 0
     _check(orc, h, "gwfq slow path")
+
+
+@pytest.mark.parametrize("kind", [0, 2], ids=["gwfq", "wfq"])
+def test_wait_free_under_delay_adversary(gcq, orc, kind):
+    # The two wait-free queues with an adversary: one thread in 320 (so one
+    # lane in every fifth wave, which holds up its whole wave) stalls about
+    # 50,000 cycles between taking a ticket and touching its slot or cell,
+    # while the other waves run on and meet the stalled tickets; a patience
+    # of 2 sends operations that lose a slot to the helping slow path at
+    # once.  Every operation of every thread completes, the history is
+    # linearizable, and the step count of each operation stays bounded.
+    print(f"[adversary] kind={NAMES[kind]}", flush=True)
+    kw = dict(enq_patience=2, deq_patience=2, help_delay=4) if kind == 0 else dict(enq_patience=2)
+    with gcq.Queue(kind, capacity=8192, max_threads=4096, **kw) as q:
+        h = q.hist_run(threads=4096, ops=16, block=256, mode=0, delay_every=320, delay_cycles=50000, seed=3)
+    assert len(h) == 4096 * 16 * 2
+    _check(orc, h, f"{NAMES[kind]} adversary")
+    steps = h["steps"]
+    print(f"[adversary] slow ops {int(h['slow'].sum())}, steps p50/p99/max "
+          f"{np.percentile(steps, 50):.0f}/{np.percentile(steps, 99):.0f}/{steps.max()}", flush=True)
+    assert h["slow"].sum() > 0
+    assert steps.max() <= STEP_BOUND[kind]
