@@ -61,6 +61,18 @@ constexpr uint64_t kInit = (0xFFull << kNoteShift) | kSafeBit | kEnqBit | (uint6
 
 __host__ __device__ __forceinline__ bool cyc_behind(uint32_t a, uint32_t b) { return ((a - b) & 0xFFu) >= 0x80u; }
 
+// Note of an entry written at cycle c outside the slow path's note advance.
+// Notes and cycles are 8 bits compared modulo 256, so a note left untouched
+// while the ring turns 128 more times would read as ahead of every later
+// ticket, and the slow path could then produce into no slot for the next
+// 128 laps (the reference keeps the old note on every write,
+// gwf_ring.hpp:404-405, 446-447).  A note behind c is therefore rewritten as
+// c - 1 (still behind c, and at most one lap old); a note at or ahead of c
+// was advanced by a slow-path helper of this cycle and is kept.
+__host__ __device__ __forceinline__ uint32_t fresh_note(uint32_t note, uint32_t c) {
+  return cyc_behind(note, c) ? ((c - 1u) & 0xFFu) : note;
+}
+
 // ------------------------------------------------------------------ local words
 // Per-request local counter: raw<<2 | INC(bit1) | FIN(bit0)   gwf_queue.hpp:202-216
 constexpr uint64_t kFin = 1ull, kInc = 2ull, kFlags = 3ull;
@@ -159,7 +171,8 @@ __device__ inline void init_slice(Ring* r, const InitParams& p, bool full, uint3
   for (uint32_t pos = gid; pos < nslots; pos += stride) {
     const uint64_t t = (uint64_t)nslots + pos;
     const uint32_t j = slot_of(t, slots_log2, p.remap_log2);
-    e[j] = (full && pos < p.n) ? e_pack(0xFF, cycle_of(t, slots_log2), true, true, 0, pos) : kInit;
+    const uint32_t c = cycle_of(t, slots_log2);
+    e[j] = (full && pos < p.n) ? e_pack(c - 1u, c, true, true, 0, pos) : kInit;
   }
   Record* rs = (Record*)((char*)r + records_off(p.n));
   for (uint32_t i = gid; i < p.max_threads; i += stride) {
@@ -254,17 +267,17 @@ struct RingOps {
     const uint32_t tcyc = cyc(T), j = slot(T);
     uint64_t* const e = slots(r);
     // Speculative CAS: in steady state the slot holds the previous lap's
-    // consumed entry (fast-path note, cycle tcyc - 1, safe, Enq, index ⊥c),
+    // consumed entry (note tcyc - 2, cycle tcyc - 1, safe, Enq, index ⊥c),
     // or the initial entry on the first lap.  When it does, one CAS replaces
     // load + CAS; otherwise the CAS returns the current word and the usual
     // checks run on it.
-    uint64_t E = (T >> g.slots_log2) == 1 ? kInit : e_pack(0xFF, tcyc - 1u, true, true, 0, kBotC);
-    bool ok = cas_rlx(&e[j], E, e_pack(0xFF, tcyc, true, true, 0, idx));
+    uint64_t E = (T >> g.slots_log2) == 1 ? kInit : e_pack(tcyc - 2u, tcyc - 1u, true, true, 0, kBotC);
+    bool ok = cas_rlx(&e[j], E, e_pack(tcyc - 1u, tcyc, true, true, 0, idx));
     while (!ok) {
       pr->step();
       if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || cnt_of(ld_rlx(head(r))) <= T)))
         break;
-      ok = cas_rlx(&e[j], E, e_pack(e_note(E), tcyc, true, true, 0, idx));
+      ok = cas_rlx(&e[j], E, e_pack(fresh_note(e_note(E), tcyc), tcyc, true, true, 0, idx));
     }
     // Threshold back to 3n - 1 after the insertions (gwf_ring.hpp:409-410):
     // one store per wave, issued after every successful lane's CAS has
@@ -340,7 +353,7 @@ struct RingOps {
         *out = eidx;
         return 0;
       }
-      const uint64_t nv = (eidx >= kBot) ? e_pack(e_note(E), hcyc, e_safe(E), true, 0, kBot)
+      const uint64_t nv = (eidx >= kBot) ? e_pack(fresh_note(e_note(E), hcyc), hcyc, e_safe(E), true, 0, kBot)
                                          : (E & ~kSafeBit);  // clear IsSafe, keep the rest
       if (cyc_behind(ecyc, hcyc) && !cas_rlx(&e[j], E, nv)) {
         pr->step();
@@ -443,7 +456,7 @@ struct RingOps {
           if (!cas_rlx(&e[j], P, nv)) continue;
           return false;
         }
-        const uint64_t prod = e_pack(enote, tcyc, true, false, owner, idx);  // Enq = 0
+        const uint64_t prod = e_pack(fresh_note(enote, tcyc), tcyc, true, false, owner, idx);  // Enq = 0
         if (!cas_rlx(&e[j], P, prod)) continue;
         const uint64_t tl = T << 2;
         if (cas_once(&rr->localTail, tl, tl | kFin)) cas_once(&e[j], prod, prod | kEnqBit);
@@ -482,7 +495,7 @@ struct RingOps {
         val = (P & ~kSafeBit);
         val = (val & ~(0xFFull << kNoteShift)) | ((uint64_t)nnote << kNoteShift);
       } else {
-        val = e_pack(nnote, hcyc, e_safe(P), true, 0, kBot);
+        val = e_pack(fresh_note(nnote, hcyc), hcyc, e_safe(P), true, 0, kBot);
       }
       if (cyc_behind(ecyc, hcyc) && !cas_rlx(&e[j], P, val)) continue;
       const uint64_t T = cnt_of(ld_rlx(tail(r)));

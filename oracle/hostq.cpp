@@ -169,6 +169,11 @@ struct GwfRing {
   static bool enqb(u64 x) { return (x >> 46) & 1; }
   static u32 idx(u64 x) { return (u32)x; }
   u32 cyc_of(u64 t) const { return (u32)(t / slots) & 0xFF; }
+  // Note of an entry written at cycle c outside the slow path's note advance:
+  // kept as is under the reference rule (gwf_ring.hpp:404-405, 446-447), else
+  // a note behind c becomes c - 1, so it never falls 128 laps behind and
+  // reads as ahead (the library's rule, gwfq.hpp fresh_note).
+  u32 wnote(u32 n, u32 c) const { return ref_rule || !cyc_behind(n, c) ? n : ((c - 1) & 0xFF); }
   u32 slot(u64 t) const { return (u32)(t & (slots - 1)); }
   u64 cnt(u64 g) const { return g >> B; }
   u32 gtid(u64 g) const { return (u32)(g & nulltid); }
@@ -190,7 +195,7 @@ struct GwfRing {
     head = pg(slots, nulltid);
     thr = -1;
     if (full) {
-      for (u32 i = 0; i < n; ++i) e[slot(slots + i)] = pack(0xFF, cyc_of(slots + i), true, true, i);
+      for (u32 i = 0; i < n; ++i) e[slot(slots + i)] = pack(wnote(0xFF, cyc_of(slots + i)), cyc_of(slots + i), true, true, i);
       tail = pg(slots + n, nulltid);
       thr = tmax();
     }
@@ -237,7 +242,7 @@ struct GwfRing {
     u64 E = e[j].load();
     for (;;) {
       if (cyc_behind(cyc(E), tc) && (safe(E) || cnt(head.load()) <= T) && idx(E) >= BOT) {
-        if (!e[j].compare_exchange_strong(E, pack(note(E), tc, true, true, index))) {
+        if (!e[j].compare_exchange_strong(E, pack(wnote(note(E), tc), tc, true, true, index))) {
           p.step();
           continue;
         }
@@ -260,7 +265,7 @@ struct GwfRing {
         *out = ei;
         return 0;
       }
-      const u64 nv = ei >= BOT ? pack(note(E), hc, safe(E), true, BOT) : pack(note(E), ec, false, enqb(E), ei);
+      const u64 nv = ei >= BOT ? pack(wnote(note(E), hc), hc, safe(E), true, BOT) : pack(note(E), ec, false, enqb(E), ei);
       if (cyc_behind(ec, hc) && !e[j].compare_exchange_strong(E, nv)) {
         p.step();
         continue;
@@ -349,12 +354,12 @@ struct GwfRing {
           if (!e[j].compare_exchange_strong(P, pack(tc, ec, safe(P), enqb(P), ei))) continue;
           return false;
         }
-        const u64 prod = pack(en, tc, true, false, index);
+        const u64 prod = pack(wnote(en, tc), tc, true, false, index);
         if (!e[j].compare_exchange_strong(P, prod)) continue;
         u64 tl = T << 2;
         if (r.localTail.compare_exchange_strong(tl, (T << 2) | FIN)) {
           u64 x = prod;
-          e[j].compare_exchange_strong(x, pack(en, tc, true, true, index));
+          e[j].compare_exchange_strong(x, pack(wnote(en, tc), tc, true, true, index));
         }
         if (thr.load() != tmax()) thr.exchange(tmax());
         return true;
@@ -375,7 +380,7 @@ struct GwfRing {
         return true;
       }
       u32 nn = en;
-      u64 val = pack(nn, hc, safe(P), true, BOT);
+      u64 val = pack(wnote(nn, hc), hc, safe(P), true, BOT);
       if (ei < BOT) {
         if (cyc_behind(ec, hc) && cyc_behind(en, hc)) {
           if (!e[j].compare_exchange_strong(P, pack(hc, ec, safe(P), enqb(P), ei))) continue;

@@ -41,3 +41,21 @@ def test_host_tb_conserves_items(orc):
         # what was dequeued in the window was enqueued in it or queued before it
         assert r["deq_success"] <= r["enq_success"] + 1024, NAMES[kind]
         assert r["succ_mops"] > 0
+
+
+def test_host_gwfq_notes_stay_fresh_past_128_laps(orc):
+    # A 16-item ring (32 slots) driven through about 375 laps with every
+    # operation on the slow path (patience 1).  Notes are 8-bit cycles
+    # compared modulo 256.  Under the reference's rule the fast path and the
+    # dequeuers' marks keep an entry's old note, so once the ring has turned
+    # 128 times the initial notes (0xFF) read as ahead of every ticket and a
+    # slow-path enqueue is turned away by every slot until the ring turns
+    # another 128 times: thousands of steps for one operation.  With the
+    # library's rule (a note behind the written cycle becomes cycle - 1) every
+    # operation stays within a few steps.
+    kw = dict(threads=2, ops=6000, mode=0, capacity=16, enq_patience=1, deq_patience=1)
+    ref_steps = max(orc.hist_run(0, ref_slow_rule=True, **kw)[1] for _ in range(3))
+    h, steps = orc.hist_run(0, **kw)
+    assert ref_steps > 32 * 64
+    assert steps < 100
+    _check(orc, h, "host gwfq past 128 laps")

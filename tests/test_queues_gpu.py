@@ -118,3 +118,19 @@ def test_wait_free_under_delay_adversary(gcq, orc, kind):
           f"{np.percentile(steps, 50):.0f}/{np.percentile(steps, 99):.0f}/{steps.max()}", flush=True)
     assert h["slow"].sum() > 0
     assert steps.max() <= STEP_BOUND[kind]
+
+
+def test_gwfq_slow_path_past_128_laps(gcq, orc):
+    # G-WFQ notes are 8-bit cycles compared modulo 256: a 64-item ring (128
+    # slots) driven through about 500 laps with every operation on the slow
+    # path.  Every operation completes in a few steps and the history is
+    # linearizable (a note left 128 laps behind would turn slow-path enqueues
+    # away from every slot for the next 128 laps).
+    print("[gwfq laps] capacity 64, 64 threads, 1000 ops, patience 1", flush=True)
+    with gcq.Queue(0, capacity=64, max_threads=64, enq_patience=1, deq_patience=1, help_delay=4) as q:
+        h = q.hist_run(threads=64, ops=1000, block=64, mode=0)
+    assert len(h) == 64 * 1000 * 2
+    assert h["slow"].sum() > 0
+    print(f"[gwfq laps] max steps {h['steps'].max()}", flush=True)
+    assert h["steps"].max() < 2000
+    _check(orc, h, "gwfq past 128 laps")
