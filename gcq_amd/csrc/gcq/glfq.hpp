@@ -190,21 +190,34 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
 }
 
 // Dequeue a slot index, or kBot when the ring is empty.
-// idle_check: consumer side, tickets bounded by the items present; the
-// free-index ring of the indirection is rarely empty and takes plain
-// aggregated tickets.
+// idle_check: consumer side, tickets bounded by the items present.  The
+// free-index ring of the indirection (idle_check false) takes plain
+// aggregated tickets while it has indices and bounded ones after this wave
+// last found it empty, so producers of a full queue do not mark free-ring
+// slots ahead of the consumers returning indices.  The per-wave hint only
+// picks the ticket path; its initial LDS contents do not matter.
 template <class Probe = NoProbe>
 __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check = true) {
   const Geometry g = r->g;
   uint64_t* const e = slots(r);
-  if (!idle_check && ld_rlx(thr(r)) < 0) return kBot;  // consumer side: the claim tests emptiness
+  __shared__ uint8_t s_low[16];  // per wave: the free ring was found empty lately
+  const uint32_t wv = ((threadIdx.x + threadIdx.y * blockDim.x) / kWave) & 15u;
+  bool bounded = idle_check;
+  if (!idle_check) {
+    if (ld_rlx(thr(r)) < 0) {  // consumer side: the claim tests emptiness
+      s_low[wv] = 1;
+      return kBot;
+    }
+    bounded = s_low[wv] != 0;
+  }
   if (idle_check) idle_pause(g.poll_backoff);
   for (;;) {
     uint64_t H;
-    if (idle_check) {
+    if (bounded) {
       H = claim_head(r);
       if (H == kEmptyTicket) {
-        idle_note(g.poll_backoff, true);
+        if (idle_check) idle_note(g.poll_backoff, true);
+        else s_low[wv] = 1;
         return kBot;
       }
       if (H == kNoTicket) {
@@ -220,7 +233,7 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
     uint64_t E = ld_rlx(&e[j]);
     // A bounded ticket lies below Tail: its enqueuer is in flight; give it a
     // moment before marking the slot.
-    for (uint32_t w = 0; idle_check && w < kSlotWaitSpins && cyc_behind(e_cyc(E), hcyc) && e_idx(E) >= kBot; ++w) {
+    for (uint32_t w = 0; bounded && w < kSlotWaitSpins && cyc_behind(e_cyc(E), hcyc) && e_idx(E) >= kBot; ++w) {
       __builtin_amdgcn_s_sleep(1);
       E = ld_rlx(&e[j]);
       if (pr) pr->step();
@@ -230,6 +243,7 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
       if (ecyc == hcyc && eidx < kBot) {
         or_rlx(&e[j], (uint64_t)kBotC);  // consume: index := ⊥c, keep cycle/safe
         if (idle_check) idle_note(g.poll_backoff, false);
+        else s_low[wv] = 0;
         return eidx;
       }
       const uint64_t nv = (eidx >= kBot) ? e_pack(hcyc, e_safe(E), kBot) : e_pack(ecyc, false, eidx);
@@ -243,9 +257,13 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
     if (T <= H + 1) {
       catchup(r, H + 1);
       faa_rlx(thr(r), -1);
+      if (!idle_check) s_low[wv] = 1;
       return kBot;
     }
-    if (faa_rlx(thr(r), -1) <= 0) return kBot;
+    if (faa_rlx(thr(r), -1) <= 0) {
+      if (!idle_check) s_low[wv] = 1;
+      return kBot;
+    }
   }
 }
 

@@ -594,20 +594,36 @@ struct RingOps {
 
   // Dequeue_wf (gwf_ring.hpp:1287-1356).  Returns an index or kBot.
   // idle_check: consumer-side dequeue with tickets bounded by the items
-  // present (claim_head); the free-index ring of the indirection is rarely
-  // empty and takes plain aggregated tickets.
+  // present (claim_head).  The free-index ring of the indirection (idle_check
+  // false) takes plain aggregated tickets while it has indices, and bounded
+  // ones after this wave last found it empty: when the queue runs full,
+  // producers taking plain tickets on an empty free ring would mark its slots
+  // ahead of the consumers returning indices and send every one of those
+  // round again (throughput fell from ~1,000 to ~6 Mops/s at 16,384 threads,
+  // split 50/50, on a 65,536-item ring).
   __device__ uint32_t dequeue(bool idle_check = true) {
     // The consumer side's claim reads Head/Tail itself; the threshold test is
-    // the quick "full" answer for the free-index ring.
-    if (!idle_check && ld_rlx(thr(r)) < 0) return kBot;
+    // the quick "full" answer for the free-index ring.  The per-wave hint is
+    // only a choice of ticket path, so its initial LDS contents do not matter.
+    __shared__ uint8_t s_low[16];  // per wave: the free ring was found empty lately
+    const uint32_t wv = ((threadIdx.x + threadIdx.y * blockDim.x) / kWave) & 15u;
+    bool bounded = idle_check;
+    if (!idle_check) {
+      if (ld_rlx(thr(r)) < 0) {
+        s_low[wv] = 1;
+        return kBot;
+      }
+      bounded = s_low[wv] != 0;
+    }
     if (idle_check) idle_pause(g.poll_backoff);
     help_threads();
     uint32_t out = kBot;
     uint64_t H = 0, last = 0;
     for (uint32_t c = g.deq_patience; --c != 0;) {
-      H = try_deq(&out, idle_check);
+      H = try_deq(&out, bounded);
       if (H == 0) {
         if (idle_check) idle_note(g.poll_backoff, out == kBot);
+        else s_low[wv] = out == kBot ? 1 : 0;
         return out;
       }
       if (H != kNoTicket) last = H;

@@ -134,3 +134,19 @@ def test_gwfq_slow_path_past_128_laps(gcq, orc):
     print(f"[gwfq laps] max steps {h['steps'].max()}", flush=True)
     assert h["steps"].max() < 2000
     _check(orc, h, "gwfq past 128 laps")
+
+
+@pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
+def test_full_queue_keeps_moving(gcq, kind):
+    # Split 50/50 and 75/25 at 16,384 threads on a 65,536-item ring keep the
+    # queue near full.  Producers then find the free-index ring empty; they
+    # must not take plain tickets there (marking its slots ahead of the
+    # consumers returning indices), which once stretched a 128-op launch from
+    # ~3 ms to ~130 ms.  Items are conserved and launches stay short.
+    for pp in (50, 75):
+        print(f"[full queue] kind={NAMES[kind]} pp={pp}", flush=True)
+        with gcq.Queue(kind, capacity=65536, max_threads=65536, poll_backoff=0) as q:
+            r = q.tb_run(1, threads=16384, chunk_ops=128, producer_percent=pp, warmup_launches=5, timed_launches=20)
+        print(f"[full queue] succ {r['succ_mops']:.1f} Mops/s, launch {r['kernel_ms_mean']:.2f} ms", flush=True)
+        assert r["deq_success"] <= r["enq_success"] + 65536
+        assert r["kernel_ms_mean"] < 25.0
