@@ -73,6 +73,30 @@ __device__ __forceinline__ uint32_t sfq_take(sfq_queue* q, uint32_t j, uint32_t 
   gcq::st_rlx(&q->ids[j], id + 2u);
   return item;
 }
+// Non-blocking claim on *ctr for the wave's active lanes: lane r tests ticket
+// t0 + r (t0 read once by the leader); one CAS claims the ready prefix
+// [t0, t0 + k).  Returns the lane's ticket, or ~0 when its op is BUSY.  The
+// reference has every lane CAS the same counter value, so at most one lane
+// of all contenders wins per round (sfqueue_hip.cpp:89-128).
+__device__ __forceinline__ uint64_t sfq_nb_claim(uint64_t* ctr, const uint32_t* ids, uint32_t add) {
+  const uint64_t m = gcq::active_mask();
+  const uint32_t leader = gcq::first_lane(m);
+  uint64_t t0 = 0;
+  if (gcq::lane_id() == leader) t0 = gcq::ld_rlx(ctr);
+  t0 = gcq::bcast64(t0, leader);
+  const uint32_t rank = gcq::rank_in(m);
+  const uint64_t t = t0 + rank;
+  const bool ready = gcq::ld_rlx(&ids[sfq_slot(t)]) == sfq_lap_id(t) + add;
+  const uint64_t not_ready = m & ~__ballot(ready);
+  const uint32_t k = (uint32_t)__builtin_popcountll(not_ready ? (m & ((1ull << gcq::first_lane(not_ready)) - 1ull)) : m);
+  uint32_t won = 0;
+  if (gcq::lane_id() == leader && k > 0) {
+    uint64_t expect = t0;
+    won = gcq::cas_rlx(ctr, expect, t0 + k) ? 1u : 0u;
+  }
+  won = gcq::bcast32(won, leader);
+  return (won && rank < k) ? t : ~0ull;
+}
 }  // namespace gcq_dropin
 
 __device__ inline int sfq_is_closed(const sfq_queue* q) { return gcq::ld_rlx(&q->closed) != 0u; }
@@ -80,7 +104,7 @@ __device__ inline void sfq_close(sfq_queue* q) { gcq::st_rlx(&q->closed, 1u); }
 
 __device__ inline int sfq_enqueue_blocking_u32(sfq_queue* q, uint32_t item) {
   if (sfq_is_closed(q)) return SFQ_CLOSED;
-  const uint64_t t = gcq::faa_rlx(&q->tail, (uint64_t)1);
+  const uint64_t t = gcq::wave_faa(&q->tail, (uint64_t)1);  // one FAA per wave
   const uint32_t j = gcq_dropin::sfq_slot(t), id = gcq_dropin::sfq_lap_id(t);
   while (gcq::ld_rlx(&q->ids[j]) != id) gcq_dropin::sfq_backoff();
   gcq_dropin::sfq_fill(q, j, id, item);
@@ -88,7 +112,7 @@ __device__ inline int sfq_enqueue_blocking_u32(sfq_queue* q, uint32_t item) {
 }
 __device__ inline int sfq_dequeue_blocking_u32(sfq_queue* q, uint32_t* out) {
   if (sfq_is_closed(q)) return SFQ_CLOSED;
-  const uint64_t h = gcq::faa_rlx(&q->head, (uint64_t)1);
+  const uint64_t h = gcq::wave_faa(&q->head, (uint64_t)1);  // one FAA per wave
   const uint32_t j = gcq_dropin::sfq_slot(h), id = gcq_dropin::sfq_lap_id(h);
   while (gcq::ld_rlx(&q->ids[j]) != id + 1u) gcq_dropin::sfq_backoff();
   *out = gcq_dropin::sfq_take(q, j, id);
@@ -97,18 +121,16 @@ __device__ inline int sfq_dequeue_blocking_u32(sfq_queue* q, uint32_t* out) {
 // Non-blocking: take the ticket only if its slot is ready (else BUSY).
 __device__ inline int sfq_enqueue_nb_u32(sfq_queue* q, uint32_t item) {
   if (sfq_is_closed(q)) return SFQ_CLOSED;
-  uint64_t t = gcq::ld_rlx(&q->tail);
-  const uint32_t j = gcq_dropin::sfq_slot(t), id = gcq_dropin::sfq_lap_id(t);
-  if (gcq::ld_rlx(&q->ids[j]) != id || !gcq::cas_rlx(&q->tail, t, t + 1)) return SFQ_BUSY;
-  gcq_dropin::sfq_fill(q, j, id, item);
+  const uint64_t t = gcq_dropin::sfq_nb_claim(&q->tail, q->ids, 0u);
+  if (t == ~0ull) return SFQ_BUSY;
+  gcq_dropin::sfq_fill(q, gcq_dropin::sfq_slot(t), gcq_dropin::sfq_lap_id(t), item);
   return SFQ_SUCCESS;
 }
 __device__ inline int sfq_dequeue_nb_u32(sfq_queue* q, uint32_t* out) {
   if (sfq_is_closed(q)) return SFQ_CLOSED;
-  uint64_t h = gcq::ld_rlx(&q->head);
-  const uint32_t j = gcq_dropin::sfq_slot(h), id = gcq_dropin::sfq_lap_id(h);
-  if (gcq::ld_rlx(&q->ids[j]) != id + 1u || !gcq::cas_rlx(&q->head, h, h + 1)) return SFQ_BUSY;
-  *out = gcq_dropin::sfq_take(q, j, id);
+  const uint64_t h = gcq_dropin::sfq_nb_claim(&q->head, q->ids, 1u);
+  if (h == ~0ull) return SFQ_BUSY;
+  *out = gcq_dropin::sfq_take(q, gcq_dropin::sfq_slot(h), gcq_dropin::sfq_lap_id(h));
   return SFQ_SUCCESS;
 }
 
