@@ -55,14 +55,21 @@ __device__ inline void fq_init(fq_mpmc_t* q) {
 __device__ inline void fq_enqueue(fq_ring_t* r, uint32_t index) { gcq::lfq::enqueue((gcq::lfq::Ring*)r, index); }
 __device__ inline uint32_t fq_dequeue(fq_ring_t* r) {
   const uint32_t x = gcq::lfq::dequeue((gcq::lfq::Ring*)r, (gcq::lfq::NoProbe*)nullptr, true);
-  return x == gcq::lfq::kBot ? FQ_BOT : x;
+  if (x == gcq::lfq::kBot) return FQ_BOT;
+  gcq::dropin_acquire();
+  return x;
 }
 
 __device__ inline bool fq_enqueue_ptr(fq_mpmc_t* q, uint64_t v) {
   return gcq::lfq::enqueue_ptr(gcq::lfq::Mpmc{(gcq::lfq::Ring*)&q->aq, (gcq::lfq::Ring*)&q->fq, q->data}, v);
 }
 __device__ inline bool fq_dequeue_ptr(fq_mpmc_t* q, uint64_t* out) {
-  return gcq::lfq::dequeue_ptr(gcq::lfq::Mpmc{(gcq::lfq::Ring*)&q->aq, (gcq::lfq::Ring*)&q->fq, q->data}, out);
+  // The reference fences after taking the value (glf_queue.hpp:579), which
+  // also orders the caller's later loads after the dequeue.
+  if (!gcq::lfq::dequeue_ptr(gcq::lfq::Mpmc{(gcq::lfq::Ring*)&q->aq, (gcq::lfq::Ring*)&q->fq, q->data}, out))
+    return false;
+  gcq::dropin_acquire();
+  return true;
 }
 
 #endif  // GCQ_DROPIN_GLF_QUEUE_HPP

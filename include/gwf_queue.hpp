@@ -109,7 +109,9 @@ template <typename Config, uint32_t N_PARAM>
 __device__ uint32_t Dequeue_wf(wf_t<Config, N_PARAM>* q) {
   gcq::gwf::NoProbe np;
   const uint32_t x = gcq::gwf::RingOps<>((gcq::gwf::Ring*)q, wf_tid(), &np).dequeue(true);
-  return x == gcq::gwf::kBot ? wf_entry::BOT : x;
+  if (x == gcq::gwf::kBot) return wf_entry::BOT;
+  gcq::dropin_acquire();
+  return x;
 }
 
 template <typename Config, uint32_t N_PARAM>
@@ -119,7 +121,11 @@ __device__ bool Enqueue_Ptr(wf_mpmc_t<Config, N_PARAM>* q, uint64_t value) {
 
 template <typename Config, uint32_t N_PARAM>
 __device__ bool Dequeue_Ptr(wf_mpmc_t<Config, N_PARAM>* q, uint64_t* out) {
-  return gcq::gwf::dequeue_ptr(gcq_dropin::gwf_view(q), wf_tid(), out);
+  // The reference fences after taking the value (gwf_ring.hpp:1378), which
+  // also orders the caller's later loads after the dequeue.
+  if (!gcq::gwf::dequeue_ptr(gcq_dropin::gwf_view(q), wf_tid(), out)) return false;
+  gcq::dropin_acquire();
+  return true;
 }
 
 #endif  // GCQ_DROPIN_GWF_QUEUE_HPP

@@ -116,6 +116,7 @@ __device__ inline int sfq_dequeue_blocking_u32(sfq_queue* q, uint32_t* out) {
   const uint32_t j = gcq_dropin::sfq_slot(h), id = gcq_dropin::sfq_lap_id(h);
   while (gcq::ld_rlx(&q->ids[j]) != id + 1u) gcq_dropin::sfq_backoff();
   *out = gcq_dropin::sfq_take(q, j, id);
+  gcq::dropin_acquire();
   return SFQ_SUCCESS;
 }
 // Non-blocking: take the ticket only if its slot is ready (else BUSY).
@@ -131,6 +132,7 @@ __device__ inline int sfq_dequeue_nb_u32(sfq_queue* q, uint32_t* out) {
   const uint64_t h = gcq_dropin::sfq_nb_claim(&q->head, q->ids, 1u);
   if (h == ~0ull) return SFQ_BUSY;
   *out = gcq_dropin::sfq_take(q, gcq_dropin::sfq_slot(h), gcq_dropin::sfq_lap_id(h));
+  gcq::dropin_acquire();
   return SFQ_SUCCESS;
 }
 

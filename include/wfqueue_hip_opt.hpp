@@ -62,7 +62,9 @@ __device__ inline bool wf_enqueue(wf_queue* q, wf_handle* h, uint64_t v) {
 }
 __device__ inline uint64_t wf_dequeue(wf_queue* q, wf_handle* h) {
   gcq::ymc::Handle* hs = (gcq::ymc::Handle*)q->q.g.ext_handles;
-  return gcq::ymc::dequeue(&q->q, hs, (uint32_t)((gcq::ymc::Handle*)h - hs));
+  const uint64_t v = gcq::ymc::dequeue(&q->q, hs, (uint32_t)((gcq::ymc::Handle*)h - hs));
+  if (v != WF_EMPTY) gcq::dropin_acquire();
+  return v;
 }
 
 namespace gcq_dropin {

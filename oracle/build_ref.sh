@@ -59,12 +59,14 @@ build_one() {  # flavour harness queue-index
   bin=$OUT/${flav}_${h}_${TAGS[$qi]}
   log=$OUT/build_${flav}_${h}_${TAGS[$qi]}.log
   if [ "$flav" = ref ]; then src=$REF/$rel; else src=$DS/$rel; fi
+  local extra=""
+  [ "$flav" = dropinrlx ] && extra="-DGCQ_DROPIN_RELAXED"
   for d in "$REF/$rel" "$ROOT"/include/*.hpp "$ROOT"/gcq_amd/csrc/gcq/*.hpp; do
     t=$(stat -L -c %Y "$d"); [ "$t" -gt "$newest" ] && newest=$t
   done
   if [ -x "$bin" ] && [ "$(stat -c %Y "$bin")" -ge "$newest" ]; then return 0; fi
   # shellcheck disable=SC2086
-  if ! $HIPCC $BASE -D${DEFS[$qi]} $(knobs_of "$h") -I"$ROOT/gcq_amd/csrc" "$src" -o "$bin" > "$log" 2>&1; then
+  if ! $HIPCC $BASE -D${DEFS[$qi]} $extra $(knobs_of "$h") -I"$ROOT/gcq_amd/csrc" "$src" -o "$bin" > "$log" 2>&1; then
     echo "build_ref: FAILED $flav $h ${TAGS[$qi]} (oracle/_ref/$(basename "$log"))"
     return 1
   fi
@@ -72,6 +74,7 @@ build_one() {  # flavour harness queue-index
 
 items=()
 for h in tb hist bfs rt; do for qi in 0 1 2 3; do items+=("dropin $h $qi"); done; done
+for qi in 0 1 2 3; do items+=("dropinrlx rt $qi"); done  # drop-in headers without the dequeue acquire
 for spec in "tb 0" "tb 1" "tb 2" "tb 3" "hist 0" "hist 1" "hist 2" "hist 3" "bfs 0" "bfs 1" "rt 0" "rt 1"; do
   items+=("ref $spec")
 done
@@ -84,5 +87,5 @@ for it in "${items[@]}"; do
   while [ "$(jobs -rp | wc -l)" -ge "${GCQ_BUILD_JOBS:-8}" ]; do sleep 0.2; done
 done
 for p in "${pids[@]}"; do wait "$p" || status=1; done
-echo "build_ref: $(ls "$OUT" | grep -cE '^(ref|dropin)_[a-z]+_[a-z]+$') binaries in oracle/_ref"
+echo "build_ref: $(ls "$OUT" | grep -cE '^(ref|dropin|dropinrlx)_[a-z]+_[a-z]+$') binaries in oracle/_ref"
 exit $status
