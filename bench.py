@@ -93,6 +93,14 @@ def main():
     # Atomic roofline denominators, same grid shape as the harness.
     p1 = g.faa_peak(threads=T, block=256, counters=1, stride_bytes=128, aggregate=True, iters=64, launches=10)
     p4 = g.faa_peak(threads=T, block=256, counters=4, stride_bytes=128, aggregate=True, iters=64, launches=10)
+    peaks = {"striped_gtickets_per_s": {}, "per_lane_1_counter_gtickets_per_s": None}
+    for k in (2, 8, 16, 64):
+        pk = g.faa_peak(threads=T, block=256, counters=k, stride_bytes=128, aggregate=True, iters=64, launches=5)
+        peaks["striped_gtickets_per_s"][str(k)] = round(pk["tickets_per_s"] / 1e9, 3)
+    peaks["striped_gtickets_per_s"]["1"] = round(p1["tickets_per_s"] / 1e9, 3)
+    peaks["striped_gtickets_per_s"]["4"] = round(p4["tickets_per_s"] / 1e9, 3)
+    raw = g.faa_peak(threads=T, block=256, counters=1, stride_bytes=128, aggregate=False, iters=4, launches=3)
+    peaks["per_lane_1_counter_gtickets_per_s"] = round(raw["tickets_per_s"] / 1e9, 4)
 
     # Consumers poll without sleeping between empty attempts (poll_backoff 0):
     # measured fastest for this split (2129 Mops/s against 2001 and 1936 at
@@ -134,6 +142,7 @@ def main():
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved_ops * BYTES_PER_OP / 1e9 / HBM_PEAK_GBS, 5),
                          "bytes_per_op": BYTES_PER_OP},
+        "atomic_peaks": peaks,
         "detail": {"enq_success": r["enq_success"], "deq_success": r["deq_success"], "deq_empty": r["deq_empty"],
                    "enq_fail": r["enq_fail"], "kernel_ms_mean": round(kernel_ms, 4),
                    "elapsed_ms": round(r["elapsed_ms"], 3), "device": info["name"], "cus": info["cus"]},
@@ -186,29 +195,42 @@ def secondary(g, with_cpu):
             cpu_s = time.perf_counter() - t
             o["cpu_baseline"] = {"value": round(es / cpu_s / 1e6, 2), "unit": "MTEPS", "cores": 1, "kind": "port",
                                  "sample": "one serial textbook BFS (oracle) on the same CSR"}
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            t = time.perf_counter()
+            orc.bfs_parallel(off, idx, 0, threads=cores)
+            par_s = time.perf_counter() - t
+            o["cpu_baseline_parallel"] = {"value": round(es / par_s / 1e6, 2), "unit": "MTEPS", "cores": cores,
+                                          "kind": "port",
+                                          "sample": "one level-synchronous std::thread BFS (oracle) on the same CSR"}
     out["bfs_lattice"] = o
     _progress(f"lattice BFS {o['mteps']} MTEPS")
 
     # --- batched vs plain (256 R-MAT queries)
     graphs, gids, srcs = bfs.rmat_query_set()
+    plain_threads = 32768
     for gr in graphs:
-        gr.bfs(0, kind=g.GLFQ, threads=8192, depths=False)
+        gr.bfs(0, kind=g.GLFQ, threads=plain_threads, depths=False)
     t = time.perf_counter()
     for gi, sv in zip(gids, srcs):
-        graphs[gi].bfs(int(sv), kind=g.GLFQ, threads=8192, depths=False)
+        graphs[gi].bfs(int(sv), kind=g.GLFQ, threads=plain_threads, depths=False)
     plain_s = time.perf_counter() - t
     with bfs.Batch(graphs) as b:
         cfg = bfs.batch_config(threads=262144, block=512)
         b.run(gids, srcs, cfg=cfg)
         ts = []
         for _ in range(5):
-            _, _, res = b.run(gids, srcs, cfg=cfg)
+            _, per, res = b.run(gids, srcs, cfg=cfg)
             ts.append(res["total_ms"])
         bms = float(np.median(ts))
+    # 28 B per reached vertex + 8 B per scanned edge, summed over the queries
+    alg = sum(28 * q["reached"] + 8 * q["edges_scanned"] for q in per)
     out["batched_bfs"] = {
         "workload": "256 R-MAT queries, scales 12..20 uniform, ef 16, seed 11, G-LFQ",
         "queries_per_s": round(256 / bms * 1e3, 1), "batch_ms_median": round(bms, 2),
         "plain_queries_per_s": round(256 / plain_s, 1), "vs_plain": round(plain_s * 1e3 / bms, 2),
+        "plain_threads": plain_threads,
+        "roofline": {"bound": "hbm", "achieved": round(alg / bms / 1e6, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / bms / 1e6 / HBM_PEAK_GBS, 5), "bytes_per_launch": alg},
         "config": {k: getattr(cfg, k) for k in ("threads", "block", "pop_width", "flush_at", "use_graph")},
         "replayed_graph": bool(res["graph_replayed"])}
     for gr in graphs:

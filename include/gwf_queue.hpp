@@ -121,7 +121,7 @@ __device__ bool Enqueue_Ptr(wf_mpmc_t<Config, N_PARAM>* q, uint64_t value) {
 
 template <typename Config, uint32_t N_PARAM>
 __device__ bool Dequeue_Ptr(wf_mpmc_t<Config, N_PARAM>* q, uint64_t* out) {
-  // The reference fences after taking the value (gwf_ring.hpp:1378), which
+  // The reference fences after taking the value (gwf_ring.hpp:1377), which
   // also orders the caller's later loads after the dequeue.
   if (!gcq::gwf::dequeue_ptr(gcq_dropin::gwf_view(q), wf_tid(), out)) return false;
   gcq::dropin_acquire();
