@@ -53,6 +53,9 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
           if (F::kQueues == 3) fq.idle(L).rewind();
         }))
       break;
+    // Drop this CU's and XCD's cached lines so the cached visited probes of
+    // the next level see at least every bit set up to this barrier.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const uint32_t n = gcq::ld_rlx(&ctl->next[L % 3]);
     if (blockIdx.x == 0) {
       if (threadIdx.x == 0) gcq::st_rlx(&ctl->next[(L + 2) % 3], 0u);
@@ -113,6 +116,9 @@ struct gcq_batch {
   gcqf::FrontierSet fs;
   int32_t* depth = nullptr;
   uint64_t depth_cap = 0;
+  uint32_t* visited = nullptr;  // per-query bitmaps, each on its own 128-B lines
+  uint64_t visited_cap = 0;     // words
+  uint64_t visited_used = 0;    // words the current batch uses (a multiple of 32)
   uint32_t slot_cap = 0;
   bk::Slot* d_slots = nullptr;
   bk::Slot* h_slots = nullptr;  // pinned
@@ -152,7 +158,15 @@ void drop_graph(gcq_batch* b) {
   b->key.clear();
 }
 
-int ensure_buffers(gcq_batch* b, uint32_t nq, uint64_t total_v) {
+int ensure_buffers(gcq_batch* b, uint32_t nq, uint64_t total_v, uint64_t visited_words) {
+  if (visited_words > b->visited_cap) {
+    drop_graph(b);
+    if (b->visited) (void)hipFree(b->visited);
+    b->visited = nullptr;
+    b->visited_cap = 0;
+    GCQ_HIP(hipMalloc(&b->visited, 4ull * visited_words));
+    b->visited_cap = visited_words;
+  }
   if (total_v > b->depth_cap) {
     drop_graph(b);
     if (b->depth) (void)hipFree(b->depth);
@@ -192,6 +206,8 @@ int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq, const bk::Dev& d, 
   hipStream_t s = b->stream;
   const uint32_t nq = d.nslots;
   hipLaunchKernelGGL(gcqf::kf_fill_i32, dim3(gcqf::grid_of(total_v)), dim3(256), 0, s, b->depth, total_v, -1);
+  hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(b->visited_used / 2)), dim3(256), 0, s,
+                     (uint64_t*)b->visited, b->visited_used / 2);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(2ull * bk::kAliveStride * nq)), dim3(256), 0, s,
                      (uint64_t*)b->alive, (uint64_t)bk::kAliveStride * nq / 2);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(2ull * nq)), dim3(256), 0, s, b->stats, 2ull * nq);
@@ -227,9 +243,9 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
               nq, 1ull << (31 - b->vbits), b->vbits);
   std::memset(res, 0, sizeof *res);
   const auto t0 = std::chrono::steady_clock::now();
-  uint64_t total_v = 0;
+  uint64_t total_v = 0, total_w = 0;
   uint32_t max_nv = 0;
-  std::vector<uint64_t> at(nq);
+  std::vector<uint64_t> at(nq), wat(nq);
   for (int32_t i = 0; i < nq; ++i) {
     GCQ_REQUIRE(gids[i] >= 0 && gids[i] < (int32_t)b->graphs.size(), "query %d: bad graph id %d", i, gids[i]);
     gcq_graph* g = b->graphs[gids[i]];
@@ -237,9 +253,12 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
     if (rc) return rc;
     at[i] = total_v;
     total_v += (uint64_t)g->host.nv;
+    wat[i] = total_w;
+    total_w += ((uint64_t)g->host.nv + 1023) / 1024 * 32;  // whole 128-B lines
     max_nv = std::max<uint32_t>(max_nv, (uint32_t)g->host.nv);
   }
-  rc = ensure_buffers(b, (uint32_t)nq, total_v);
+  rc = ensure_buffers(b, (uint32_t)nq, total_v, total_w);
+  b->visited_used = total_w;
   if (rc) return rc;
   const uint64_t need = total_v;
   uint64_t cap_items = cfg->capacity_log2 ? (1ull << cfg->capacity_log2) : need;
@@ -255,6 +274,7 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
     s.off = (uint64_t)g->d_off;
     s.idx = (uint64_t)g->d_idx;
     s.depth = (uint64_t)(b->depth + at[i]);
+    s.visited = (uint64_t)(b->visited + wat[i]);
     s.start = 0;
     s.src = (srcs[i] >= 0 && srcs[i] < g->host.nv) ? (uint32_t)srcs[i] : 0u;  // bfs_bench.cpp:555
     s.nv = (uint32_t)g->host.nv;
@@ -284,7 +304,8 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
   if (cfg->use_graph) {
     const std::vector<uint64_t> key = {(uint64_t)cfg->kind, cfg->threads, cfg->block, cfg->pop_width, cfg->flush_at,
                                        cfg->capacity_log2, (uint64_t)nq, total_v, max_nv,
-                                       (uint64_t)b->fs.q[0], (uint64_t)b->depth, (uint64_t)b->d_slots};
+                                       (uint64_t)b->fs.q[0], (uint64_t)b->depth, (uint64_t)b->d_slots,
+                                       (uint64_t)b->visited, total_w};
     if (key != b->key || !b->exec) {
       drop_graph(b);
       GCQ_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -504,6 +525,7 @@ extern "C" int gcq_batch_destroy(gcq_batch* b) {
   b->fs.release();
   free_slots(b);
   if (b->depth) (void)hipFree(b->depth);
+  if (b->visited) (void)hipFree(b->visited);
   if (b->ctl) (void)hipFree(b->ctl);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -18,7 +18,8 @@ struct Slot {
   uint64_t off, idx, depth;  // CSR of its graph, its depth array
   uint32_t start, src;       // level at which it starts; its source
   uint32_t nv, pad;
-  uint64_t pad2[3];
+  uint64_t visited;          // optional bitmap, one bit per vertex (0 = none)
+  uint64_t pad2[2];
 };
 static_assert(sizeof(Slot) == 64, "slot");
 
@@ -57,7 +58,7 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
     if (__ballot(got) == 0) break;  // every popping lane saw the level's queue empty
     uint32_t slot = 0;
     int32_t beg = 0, deg = 0, dn = 0;
-    uint64_t idxp = 0, depp = 0;
+    uint64_t idxp = 0, depp = 0, vis = 0;
     if (got) {
       slot = item >> d.vbits;
       const uint32_t v = item & vmask;
@@ -65,6 +66,7 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
       const int32_t* off = (const int32_t*)rd(&s->off);
       idxp = rd(&s->idx);
       depp = rd(&s->depth);
+      if (!kLive) vis = rd(&s->visited);
       dn = L - (int32_t)rd(&s->start) + 1;
       beg = off[v];  // CSR arrays are uploaded before launch and never written
       deg = off[v + 1] - beg;
@@ -91,18 +93,31 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
       const int32_t odn = __shfl(dn, lo, 64);
       const uint64_t oidx = shfl64(idxp, lo);
       const uint64_t odep = shfl64(depp, lo);
+      const uint64_t ovis = kLive ? 0 : shfl64(vis, lo);
       bool disc = false;
       uint32_t w = 0;
       if (e < total) {
         w = (uint32_t)((const int32_t*)oidx)[obeg + (ec - oexcl)];
         int32_t* dp = (int32_t*)odep + w;
-        if (rd(dp) == -1) {
+        // The bitmap packs 32 vertices per word where the depth array holds
+        // one, so most visited probes are served by a cached line; a bit read
+        // stale (clear) only sends the probe on to the depth array.
+        uint32_t* vw = ovis ? (uint32_t*)ovis + (w >> 5) : nullptr;
+        const uint32_t vb = 1u << (w & 31u);
+        if ((!vw || !(gcq::ld_cached(vw) & vb)) && rd(dp) == -1) {
           int32_t expect = -1;
           disc = gcq::cas_rlx(dp, expect, odn);
+          if (disc && vw) gcq::or_rlx(vw, vb);
         }
       }
-      if (disc) gcq::st_rlx(&d.alive[oslot * kAliveStride], (uint32_t)(L + 1));
       const uint64_t dm = __ballot(disc);
+      // One activity store per run of discovering lanes with the same query
+      // (neighbours of one vertex sit on consecutive lanes): a lane stores
+      // only if the nearest discovering lane below it belongs to another query.
+      const uint64_t below = dm & ((1ull << lane) - 1ull);
+      const int pl = below ? 63 - __builtin_clzll(below) : (int)lane;
+      const uint32_t pslot = __shfl(oslot, pl, 64);
+      if (disc && (!below || pslot != oslot)) gcq::st_rlx(&d.alive[oslot * kAliveStride], (uint32_t)(L + 1));
       if (disc) stage[sc + gcq::rank_in(dm)] = (oslot << d.vbits) | w;
       sc += (uint32_t)__builtin_popcountll(dm);
       disc_n += disc ? 1u : 0u;
