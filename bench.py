@@ -17,6 +17,8 @@ Beside it, measured in the same run:
 
 Unless --headline-only, the line also carries the other workloads of the
 metric, each measured in this run ("secondary"):
+  * queue_sweep: all four queues, split 25/75 and balanced, 1,024-65,536
+    threads, plus the FIFO self-check at 65,536 threads x 32 operations;
   * hbm_copy: measured streaming-copy bandwidth (the measured HBM roofline);
   * bfs_lattice: queue-driven BFS on the 4900 x 4900 lattice (keep 0.75, 1%
     diagonals, seed 7, source 0, 8,192 threads, G-LFQ, one launch), 3 warmup
@@ -169,6 +171,26 @@ def secondary(g, with_cpu):
     out["hbm_copy"] = {"gbps": round(gbps, 1), "bytes": 1 << 30, "kernel_ms": round(copy_ms, 4)}
     _progress(f"hbm copy {gbps:.0f} GB/s")
 
+    # --- all four queues: split 25/75 and balanced, threads swept, capacity 2^20
+    sweep = {}
+    names = {g.GWFQ: "gwfq", g.GLFQ: "glfq", g.WFQ: "wfq", g.SFQ: "sfq"}
+    for kind in (g.GWFQ, g.GLFQ, g.WFQ, g.SFQ):
+        rows = {}
+        extra = {"wfq_segment_cells": 1 << 28} if kind == g.WFQ else {}
+        for mode, mname in ((g.SPLIT, "split25"), (0, "balanced")):
+            for T in (1024, 4096, 16384, 65536):
+                with g.Queue(kind, capacity=1 << 20, max_threads=65536, poll_backoff=0, **extra) as q:
+                    r = q.tb_run(mode, threads=T, block=256, chunk_ops=128, producer_percent=25,
+                                 warmup_launches=5, timed_launches=20)
+                rows[f"{mname}_{T}"] = round(r["succ_mops"], 1)
+        with g.Queue(kind, capacity=1 << 20, max_threads=65536, **extra) as q:
+            f = q.fifo_check(65536, 32, concurrent=True)
+        rows["fifo_check_65536x32_verified"] = bool(f["verified"])
+        sweep[names[kind]] = rows
+    out["queue_sweep"] = {"unit": "Mops/s (successful enq + deq)", "chunk_ops": 128, "block": 256,
+                          "launches": "5 warmup + 20 timed", "capacity": 1 << 20, "queues": sweep}
+    _progress("queue sweep " + ", ".join(f"{k} {v['split25_65536']}" for k, v in sweep.items()))
+
     # --- plain BFS on the lattice
     with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as gr:
         nv, ne = gr.info()
@@ -215,7 +237,12 @@ def secondary(g, with_cpu):
         graphs[gi].bfs(int(sv), kind=g.GLFQ, threads=plain_threads, depths=False)
     plain_s = time.perf_counter() - t
     with bfs.Batch(graphs) as b:
-        cfg = bfs.batch_config(threads=262144, block=512)
+        # block, grid, pop width, flush width and ring capacity from the
+        # autotuner (coordinate descent, one timed run per trial)
+        t = time.perf_counter()
+        cfg, trials = b.autotune(gids, srcs, kind=g.GLFQ, reps=1, max_trials=64)
+        tune_s = time.perf_counter() - t
+        cfg.use_graph = 1
         b.run(gids, srcs, cfg=cfg)
         ts = []
         for _ in range(5):
@@ -231,7 +258,9 @@ def secondary(g, with_cpu):
         "plain_threads": plain_threads,
         "roofline": {"bound": "hbm", "achieved": round(alg / bms / 1e6, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / bms / 1e6 / HBM_PEAK_GBS, 5), "bytes_per_launch": alg},
-        "config": {k: getattr(cfg, k) for k in ("threads", "block", "pop_width", "flush_at", "use_graph")},
+        "config": {k: getattr(cfg, k) for k in ("threads", "block", "pop_width", "flush_at", "capacity_log2",
+                                                 "use_graph")},
+        "autotune": {"trials": len(trials), "seconds": round(tune_s, 1)},
         "replayed_graph": bool(res["graph_replayed"])}
     for gr in graphs:
         gr.close()
