@@ -1,11 +1,13 @@
 #!/usr/bin/env bash
 # ORACLE build recipe (test infrastructure only).  Compiles, where the
-# reference tree exists, two sets of binaries into oracle/_ref/ (git-ignored;
-# they travel to the GPU box with the tree):
+# reference tree exists, these binaries into oracle/_ref/ (git-ignored; they
+# are copied along with the working tree to the machine that runs them):
 #   ref_<harness>_<queue>     the reference harness with the reference's own
 #                             queue headers, straight from /root/reference;
 #   dropin_<harness>_<queue>  the same harness compiled against this
-#                             repository's drop-in headers (include/*.hpp).
+#                             repository's drop-in headers (include/*.hpp);
+#   dropinrlx_rt_<queue>      the ray tracing harness on the drop-in headers
+#                             without the dequeue acquire (GCQ_DROPIN_RELAXED).
 #                             oracle/_ref/dsrc/ mirrors the reference layout
 #                             with symbolic links only: the drop-in headers at
 #                             its top, links to the harness sources in their

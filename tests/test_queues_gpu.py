@@ -50,7 +50,7 @@ def test_split_history(gcq, orc, kind):
 
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
 def test_history_65536_threads(gcq, orc, kind):
-    # All four queues at 65,536 threads (the north-star scale); capacity 2^20.
+    # All four queues at 65,536 threads, the largest throughput configuration; capacity 2^20.
     print(f"[hist 64k] kind={NAMES[kind]}", flush=True)
     with gcq.Queue(kind, capacity=1 << 20, max_threads=65536) as q:
         h = q.hist_run(threads=65536, ops=4, block=256, mode=0)
