@@ -48,7 +48,8 @@ class FaaResult(C.Structure):
 class TbConfig(C.Structure):
     _fields_ = [("mode", C.c_int32), ("threads", C.c_uint32), ("block", C.c_uint32), ("chunk_ops", C.c_uint32),
                 ("producer_percent", C.c_uint32), ("warmup_ms", C.c_uint32), ("run_ms", C.c_uint32),
-                ("launches_per_sync", C.c_uint32), ("warmup_launches", C.c_uint32), ("timed_launches", C.c_uint32)]
+                ("launches_per_sync", C.c_uint32), ("warmup_launches", C.c_uint32), ("timed_launches", C.c_uint32),
+                ("solo_claims", C.c_uint32)]
 
 
 class TbResult(C.Structure):
@@ -199,11 +200,13 @@ class Queue:
 
     # ------------------------------------------------------------------ harnesses
     def tb_run(self, mode, threads, block=256, chunk_ops=128, producer_percent=25, warmup_ms=200, run_ms=1000,
-               launches_per_sync=0, warmup_launches=0, timed_launches=0):
+               launches_per_sync=0, warmup_launches=0, timed_launches=0, solo_claims=False):
         """Timed balanced/split run: a time window (run_ms) or, with
-        timed_launches > 0, exactly that many launches after warmup_launches."""
+        timed_launches > 0, exactly that many launches after warmup_launches.
+        solo_claims turns off the workgroup combining of idle consumers'
+        claims (G-WFQ, G-LFQ)."""
         cfg = TbConfig(mode, threads, block, chunk_ops, producer_percent, warmup_ms, run_ms, launches_per_sync,
-                       warmup_launches, timed_launches)
+                       warmup_launches, timed_launches, int(bool(solo_claims)))
         res = TbResult()
         _check(load().gcq_tb_run(self.handle, C.byref(cfg), C.byref(res)), "gcq_tb_run")
         return {f: getattr(res, f) for f, _ in TbResult._fields_}

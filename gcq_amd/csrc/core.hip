@@ -390,8 +390,12 @@ namespace {
 // (throughout_bench.cpp:836-839, 878-881).
 template <class Q>
 __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32_t pp, uint32_t launch,
-                     uint64_t* stats) {
+                     uint32_t solo, uint64_t* stats) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ gcq::Combine s_cb;
+  if (threadIdx.x == 0) gcq::combine_init(&s_cb);
+  __syncthreads();
+  gcq::Combine* const cb = solo ? nullptr : &s_cb;
   uint64_t eo = 0, ef = 0, dok = 0, de = 0;
   if (tid < threads) {
     const uint64_t seq0 = (uint64_t)launch * chunk + 1;
@@ -403,7 +407,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
           if (q.enq(tid, v)) ++eo; else ++ef;
         } else {
           uint64_t o;
-          if (q.deq(tid, &o)) ++dok; else ++de;
+          if (q.deq(tid, &o, nullptr, cb)) ++dok; else ++de;
         }
       }
     } else {
@@ -411,7 +415,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
         const uint64_t v = ((uint64_t)tid << 32) | (uint32_t)(seq0 + i);
         if (q.enq(tid, v)) ++eo; else ++ef;
         uint64_t o;
-        if (q.deq(tid, &o)) ++dok; else ++de;
+        if (q.deq(tid, &o, nullptr, cb)) ++dok; else ++de;
       }
     }
   }
@@ -457,7 +461,7 @@ extern "C" int gcq_tb_run(gcq_queue* q, const gcq_tb_config* cfg, gcq_tb_result*
         using A = decltype(a);
         auto go = [&]() {
           hipLaunchKernelGGL(k_tb<A>, dim3(grid), dim3(cfg->block), 0, q->stream, a, cfg->threads, cfg->chunk_ops,
-                             cfg->mode, cfg->producer_percent, launch++, stats);
+                             cfg->mode, cfg->producer_percent, launch++, cfg->solo_claims, stats);
         };
         using clk = std::chrono::steady_clock;
         // Warmup: host-timed like the reference (throughout_bench.cpp:1010-1021).
@@ -547,10 +551,10 @@ constexpr uint32_t kFifoSpinLimit = 1u << 22;
 // correctly ordered dequeues whose atomicMax calls land in the other order.
 template <class Q>
 __device__ void fifo_take(const Q& q, uint32_t tid, const FifoDev& f, uint32_t producers, uint32_t ops,
-                          uint64_t total, uint32_t* streak) {
+                          uint64_t total, uint32_t* streak, gcq::Combine* cb = nullptr) {
   uint64_t v = 0;
   const uint64_t call = gcq::faa_rlx(f.clock, (uint64_t)1);
-  if (!q.deq(tid, &v)) {
+  if (!q.deq(tid, &v, nullptr, cb)) {
     ++*streak;
     return;
   }
@@ -607,6 +611,9 @@ template <class Q>
 __global__ void k_fifo_mixed(Q q, uint32_t threads, uint32_t producers, FifoDev f, uint32_t ops, uint64_t total,
                              uint32_t* fail) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ gcq::Combine s_cb;
+  if (threadIdx.x == 0) gcq::combine_init(&s_cb);
+  __syncthreads();
   if (tid >= threads) return;
   if (tid < producers) {
     for (uint32_t i = 0; i < ops; ++i) {
@@ -624,7 +631,7 @@ __global__ void k_fifo_mixed(Q q, uint32_t threads, uint32_t producers, FifoDev 
   }
   uint32_t streak = 0;
   while (gcq::ld_rlx(f.consumed) < total && streak < kFifoSpinLimit) {
-    fifo_take(q, tid, f, producers, ops, total, &streak);
+    fifo_take(q, tid, f, producers, ops, total, &streak, &s_cb);
     if (streak) __builtin_amdgcn_s_sleep(2);
   }
 }
@@ -750,6 +757,9 @@ static_assert(sizeof(LogEntry) == sizeof(gcq_hist_entry), "log entry layout");
 template <class Q>
 __global__ void k_hist(Q q, gcq_hist_config c, uint32_t stride, LogEntry* log, uint64_t* clock) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ gcq::Combine s_cb;
+  if (threadIdx.x == 0) gcq::combine_init(&s_cb);
+  __syncthreads();
   if (tid >= c.threads) return;
   const bool victim = c.delay_every && ((tid + (uint32_t)c.seed) % c.delay_every == 0);
   const bool prod = ((uint64_t)tid * 100u) / c.threads < c.producer_percent;
@@ -783,7 +793,7 @@ __global__ void k_hist(Q q, gcq_hist_config c, uint32_t stride, LogEntry* log, u
       e.arg = 0;
       e.call = gcq::faa_rlx(clock, (uint64_t)1);
       uint64_t out = 0;
-      const bool ok = q.deq(tid, &out, &p);
+      const bool ok = q.deq(tid, &out, &p, &s_cb);
       e.ret_time = gcq::faa_rlx(clock, (uint64_t)1);
       e.ret = ok ? out : 0;
       e.steps = p.steps;

@@ -146,6 +146,32 @@ __device__ __forceinline__ uint64_t claim_head(Ring* r) {
   return rank < k ? base + rank : kNoTicket;
 }
 
+// claim_head() with the Head/Tail reads and the FAA shared by the waves of
+// the workgroup that claim together (combine_claim).
+__device__ __forceinline__ uint64_t claim_head_combined(Ring* r, Combine* cb) {
+  const uint64_t m = active_mask();
+  const uint32_t leader = first_lane(m);
+  const uint32_t n = (uint32_t)__builtin_popcountll(m);
+  uint64_t base = 0;
+  uint32_t k = 0, off = 0;
+  if (lane_id() == leader) {
+    off = combine_claim(cb, n, &k, &base, [&](uint32_t lanes, uint32_t* kk, uint64_t* bb) {
+      const uint64_t h = ld_rlx(head(r));
+      vm_wait();  // Head is read before Tail
+      const uint64_t t = ld_rlx(tail(r));
+      const uint64_t avail = t > h ? t - h : 0;
+      *kk = avail < lanes ? (uint32_t)avail : lanes;
+      *bb = *kk ? faa_rlx(head(r), (uint64_t)*kk) : 0;
+    });
+  }
+  k = bcast32(k, leader);
+  if (k == 0) return kEmptyTicket;
+  off = bcast32(off, leader);
+  base = bcast64(base, leader);
+  const uint32_t rank = rank_in(m);
+  return off + rank < k ? base + off + rank : kNoTicket;
+}
+
 struct NoProbe {
   __device__ __forceinline__ void step() {}
   __device__ __forceinline__ void after_ticket() {}
@@ -197,7 +223,7 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
 // slots ahead of the consumers returning indices.  The per-wave hint only
 // picks the ticket path; its initial LDS contents do not matter.
 template <class Probe = NoProbe>
-__device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check = true) {
+__device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check = true, Combine* cb = nullptr) {
   const Geometry g = r->g;
   uint64_t* const e = slots(r);
   __shared__ uint8_t s_low[16];  // per wave: the free ring was found empty lately
@@ -214,7 +240,15 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
   for (;;) {
     uint64_t H;
     if (bounded) {
-      H = claim_head(r);
+      // Combined with the workgroup's other idle waves when cb is given.
+      if (cb) {
+        uint8_t* const recent = combine_recent();
+        const uint32_t rc = *recent & 7u;
+        H = rc ? claim_head_combined(r, cb) : claim_head(r);
+        *recent = (uint8_t)(H == kEmptyTicket ? kCombineAfterEmpty : (rc ? rc - 1u : 0u));
+      } else {
+        H = claim_head(r);
+      }
       if (H == kEmptyTicket) {
         if (idle_check) idle_note(g.poll_backoff, true);
         else s_low[wv] = 1;
@@ -294,8 +328,8 @@ __device__ inline bool enqueue_ptr(const Mpmc& q, uint64_t v, Probe* pr = nullpt
 }
 
 template <class Probe = NoProbe>
-__device__ inline bool dequeue_ptr(const Mpmc& q, uint64_t* out, Probe* pr = nullptr) {
-  const uint32_t idx = dequeue(q.aq, pr);
+__device__ inline bool dequeue_ptr(const Mpmc& q, uint64_t* out, Probe* pr = nullptr, Combine* cb = nullptr) {
+  const uint32_t idx = dequeue(q.aq, pr, true, cb);
   if (idx == kBot) return false;  // empty
   *out = ld_rlx(&q.data[idx]);  // address-dependent on the dequeued index;
   enqueue(q.fq, idx, pr);        // completes before enqueue() touches a slot

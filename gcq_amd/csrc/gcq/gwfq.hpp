@@ -320,12 +320,46 @@ struct RingOps {
     return rank < k ? base + rank : kNoTicket;
   }
 
-  // bounded: take the ticket with claim_head() (consumer side); otherwise one
+  // claim_head() with the Head/Tail reads and the FAA shared by the waves of
+  // the workgroup that claim together (combine_claim).
+  __device__ __forceinline__ uint64_t claim_head_combined(Combine* cb) {
+    const uint64_t m = active_mask();
+    const uint32_t leader = first_lane(m);
+    const uint32_t n = (uint32_t)__builtin_popcountll(m);
+    uint64_t base = 0;
+    uint32_t k = 0, off = 0;
+    if (lane_id() == leader) {
+      off = combine_claim(cb, n, &k, &base, [&](uint32_t lanes, uint32_t* kk, uint64_t* bb) {
+        const uint64_t hw = ld_rlx(head(r));
+        vm_wait();  // Head is read before Tail
+        const uint64_t t = cnt_of(ld_rlx(tail(r))), h = cnt_of(hw);
+        const uint64_t avail = t > h ? t - h : 0;
+        *kk = avail < lanes ? (uint32_t)avail : lanes;
+        *bb = *kk ? cnt_of(faa_rlx(head(r), (uint64_t)*kk << g.thr_bits)) : 0;
+      });
+    }
+    k = bcast32(k, leader);
+    if (k == 0) return kEmptyTicket;
+    off = bcast32(off, leader);
+    base = bcast64(base, leader);
+    const uint32_t rank = rank_in(m);
+    return off + rank < k ? base + off + rank : kNoTicket;
+  }
+
+  // bounded: take the ticket with claim_head() (consumer side), combined
+  // with the workgroup's other idle waves when cb is given; otherwise one
   // aggregated FAA (the free-index ring of the indirection).
-  __device__ __forceinline__ uint64_t try_deq(uint32_t* out, bool bounded) {
+  __device__ __forceinline__ uint64_t try_deq(uint32_t* out, bool bounded, Combine* cb = nullptr) {
     uint64_t H;
     if (bounded) {
-      H = claim_head();
+      if (cb) {
+        uint8_t* const recent = combine_recent();
+        const uint32_t rc = *recent & 7u;
+        H = rc ? claim_head_combined(cb) : claim_head();
+        *recent = (uint8_t)(H == kEmptyTicket ? kCombineAfterEmpty : (rc ? rc - 1u : 0u));
+      } else {
+        H = claim_head();
+      }
       if (H == kEmptyTicket) {
         *out = kBot;
         return 0;
@@ -601,7 +635,7 @@ struct RingOps {
   // ahead of the consumers returning indices and send every one of those
   // round again (throughput fell from ~1,000 to ~6 Mops/s at 16,384 threads,
   // split 50/50, on a 65,536-item ring).
-  __device__ uint32_t dequeue(bool idle_check = true) {
+  __device__ uint32_t dequeue(bool idle_check = true, Combine* cb = nullptr) {
     // The consumer side's claim reads Head/Tail itself; the threshold test is
     // the quick "full" answer for the free-index ring.  The per-wave hint is
     // only a choice of ticket path, so its initial LDS contents do not matter.
@@ -620,7 +654,7 @@ struct RingOps {
     uint32_t out = kBot;
     uint64_t H = 0, last = 0;
     for (uint32_t c = g.deq_patience; --c != 0;) {
-      H = try_deq(&out, bounded);
+      H = try_deq(&out, bounded, cb);
       if (H == 0) {
         if (idle_check) idle_note(g.poll_backoff, out == kBot);
         else s_low[wv] = out == kBot ? 1 : 0;
@@ -698,10 +732,11 @@ __device__ inline bool enqueue_ptr(const Mpmc& q, uint32_t tid, uint64_t v, Prob
 }
 
 template <class Probe = NoProbe>
-__device__ inline bool dequeue_ptr(const Mpmc& q, uint32_t tid, uint64_t* out, Probe* p = nullptr) {
+__device__ inline bool dequeue_ptr(const Mpmc& q, uint32_t tid, uint64_t* out, Probe* p = nullptr,
+                                   Combine* cb = nullptr) {
   NoProbe np;
   Probe* pp = p ? p : (Probe*)&np;
-  const uint32_t idx = RingOps<Probe>(q.aq, tid, pp).dequeue();
+  const uint32_t idx = RingOps<Probe>(q.aq, tid, pp).dequeue(true, cb);
   if (idx == kBot) return false;  // empty
   *out = ld_rlx(&q.data[idx]);  // address-dependent; completes before try_enq touches a slot
   RingOps<Probe>(q.fq, tid, pp).enqueue(idx);

@@ -22,6 +22,10 @@ struct gcq_queue {
 
 namespace gcqk {
 
+// deq's Combine argument: the kernel's LDS context for combining the
+// consumer-side claims of a workgroup's idle waves (G-WFQ, G-LFQ; the other
+// queues take no bounded claim and ignore it).
+
 struct GwfQ {
   gcq::gwf::Mpmc m;
   template <class P = gcq::gwf::NoProbe>
@@ -29,8 +33,11 @@ struct GwfQ {
     return gcq::gwf::enqueue_ptr<P>(m, tid, v, p);
   }
   template <class P = gcq::gwf::NoProbe>
-  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr) const {
-    return gcq::gwf::dequeue_ptr<P>(m, tid, o, p);
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr, gcq::Combine* cb = nullptr) const {
+    return gcq::gwf::dequeue_ptr<P>(m, tid, o, p, cb);
+  }
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, decltype(nullptr), gcq::Combine* cb) const {
+    return deq(tid, o, (gcq::gwf::NoProbe*)nullptr, cb);
   }
 };
 
@@ -41,8 +48,11 @@ struct LfqQ {
     return gcq::lfq::enqueue_ptr<P>(m, v, p);
   }
   template <class P = gcq::lfq::NoProbe>
-  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr) const {
-    return gcq::lfq::dequeue_ptr<P>(m, o, p);
+  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr, gcq::Combine* cb = nullptr) const {
+    return gcq::lfq::dequeue_ptr<P>(m, o, p, cb);
+  }
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, decltype(nullptr), gcq::Combine* cb) const {
+    return deq(tid, o, (gcq::lfq::NoProbe*)nullptr, cb);
   }
 };
 
@@ -59,9 +69,12 @@ struct SfqQ {
     return gcq::sfq::enqueue_nb(q, v, p) == gcq::sfq::kSuccess;
   }
   template <class P = gcq::sfq::NoProbe>
-  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr) const {
+  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr, gcq::Combine* = nullptr) const {
     if (exact_empty) return gcq::sfq::dequeue_or_empty(q, o, 1u << 20, p);
     return gcq::sfq::dequeue_nb(q, o, p) == gcq::sfq::kSuccess;
+  }
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, decltype(nullptr), gcq::Combine* cb) const {
+    return deq(tid, o, (gcq::sfq::NoProbe*)nullptr, cb);
   }
 };
 
@@ -73,11 +86,14 @@ struct YmcQ {
     return gcq::ymc::enqueue(q, h, tid, v, p);
   }
   template <class P = gcq::ymc::NoProbe>
-  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr) const {
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr, gcq::Combine* = nullptr) const {
     const uint64_t v = gcq::ymc::dequeue(q, h, tid, p);
     if (v == gcq::ymc::kEmpty) return false;
     *o = v;
     return true;
+  }
+  __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, decltype(nullptr), gcq::Combine* cb) const {
+    return deq(tid, o, (gcq::ymc::NoProbe*)nullptr, cb);
   }
 };
 

@@ -1,0 +1,150 @@
+// CPU model of the workgroup claim-combining protocol of
+// gcq_amd/csrc/gcq/prim.hpp (combine_claim), test infrastructure only.
+//
+// Threads stand for the waves of one workgroup; std::atomic words stand for
+// the LDS words and for the queue's Head/Tail counters.  A producer thread
+// advances Tail.  Checked for every trial:
+//  - every ticket handed out is unique, and the tickets form exactly
+//    [0, Head): no ticket is lost or handed out twice;
+//  - every member of a group started its operation before the leader read
+//    Tail (an event counter taken at each wave's call and just before the
+//    leader's Tail read), so an EMPTY answer is a witness inside each
+//    member's operation;
+//  - no wave waits forever (a spin bound turns a deadlock into a failure).
+// Built with -DPLANT_LATE_CLOSE the leader reads Tail before it closes its
+// group, the mistake the ordering rule exists to prevent; the member check
+// must then fail.
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct Combine {
+  std::atomic<uint64_t> join;
+  std::atomic<uint32_t> done;
+  std::atomic<uint32_t> res_g[16];
+  uint32_t res_k[16];
+  uint64_t res_base[16];
+};
+
+std::atomic<uint64_t> g_head{0}, g_tail{0}, g_event{0};
+std::atomic<uint64_t> g_call_event[16];
+std::atomic<uint64_t> g_failures{0};
+constexpr uint64_t kSpinLimit = 1ull << 28;
+
+void fail(const char* what) {
+  if (g_failures.fetch_add(1) < 5) std::fprintf(stderr, "FAIL: %s\n", what);
+}
+
+// One claim by wave w for n lanes; out[r] = ticket, or ~0 for none.
+// Returns true when the group answered EMPTY.
+bool claim(Combine* c, uint32_t w, uint32_t n, uint64_t* out, std::mt19937& rng) {
+  g_call_event[w].store(g_event.fetch_add(1));
+  const uint64_t old = c->join.fetch_add((1ull << (32 + w)) + n);
+  const uint32_t grp = (uint32_t)(old >> 48);
+  const uint32_t off = (uint32_t)old;
+  uint32_t k = 0;
+  uint64_t base = 0;
+  if (off == 0) {
+    for (uint64_t s = 0; (c->done.load(std::memory_order_acquire) & 0xFFFFu) != ((grp - 1u) & 0xFFFFu); ++s) {
+      if (s > kSpinLimit) { fail("leader waited forever"); break; }
+      std::this_thread::yield();
+    }
+    const uint64_t h = g_head.load();
+#ifdef PLANT_LATE_CLOSE
+    const uint64_t tread = g_event.fetch_add(1);
+    const uint64_t t = g_tail.load();
+    for (int i = 0; i < 8; ++i) std::this_thread::yield();
+    const uint64_t fin = c->join.exchange((uint64_t)((grp + 1u) & 0xFFFFu) << 48);
+#else
+    const uint64_t fin = c->join.exchange((uint64_t)((grp + 1u) & 0xFFFFu) << 48);
+    if (rng() % 4 == 0) std::this_thread::yield();
+    const uint64_t tread = g_event.fetch_add(1);
+    const uint64_t t = g_tail.load();
+#endif
+    const uint32_t lanes = (uint32_t)fin, members = (uint32_t)(fin >> 32) & 0xFFFFu;
+    if (!(members & (1u << w))) fail("leader is not a member of its group");
+    for (uint32_t b = members; b; b &= b - 1)
+      if (g_call_event[__builtin_ctz(b)].load() >= tread) fail("a member started after the leader read Tail");
+    const uint64_t avail = t > h ? t - h : 0;
+    k = avail < lanes ? (uint32_t)avail : lanes;
+    if (k) base = g_head.fetch_add(k);
+    for (uint32_t b = members & ~(1u << w); b; b &= b - 1) {
+      const uint32_t x = (uint32_t)__builtin_ctz(b);
+      c->res_k[x] = k;
+      c->res_base[x] = base;
+      c->res_g[x].store(grp, std::memory_order_release);
+    }
+    c->done.store(grp, std::memory_order_release);
+  } else {
+    for (uint64_t s = 0; c->res_g[w].load(std::memory_order_acquire) != grp; ++s) {
+      if (s > kSpinLimit) { fail("member waited forever"); break; }
+      std::this_thread::yield();
+    }
+    k = c->res_k[w];
+    base = c->res_base[w];
+  }
+  for (uint32_t r = 0; r < n; ++r) out[r] = off + r < k ? base + off + r : ~0ull;
+  return k == 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const int trials = argc > 1 ? std::atoi(argv[1]) : 30;
+  const int ops = argc > 2 ? std::atoi(argv[2]) : 2000;
+  uint64_t empties = 0, groups_with_items = 0;
+  for (int trial = 0; trial < trials; ++trial) {
+    g_head = 0;
+    g_tail = 0;
+    Combine c;
+    c.join = 1ull << 48;
+    c.done = 0;
+    for (auto& x : c.res_g) x = ~0u;
+    const int waves = 2 + trial % 15;
+    std::vector<std::vector<uint64_t>> got(waves);
+    std::atomic<bool> stop{false};
+    std::thread producer([&] {
+      std::mt19937 r(trial);
+      while (!stop) {
+        g_tail.fetch_add(r() % 96);
+        for (int i = 0; i < 40; ++i) std::this_thread::yield();
+      }
+    });
+    std::vector<std::thread> th;
+    std::atomic<uint64_t> trial_empties{0};
+    for (int w = 0; w < waves; ++w)
+      th.emplace_back([&, w] {
+        std::mt19937 rng(trial * 131 + w);
+        uint64_t out[64];
+        for (int i = 0; i < ops; ++i) {
+          const uint32_t n = 1 + rng() % 64;
+          if (claim(&c, (uint32_t)w, n, out, rng)) trial_empties++;
+          for (uint32_t j = 0; j < n; ++j)
+            if (out[j] != ~0ull) got[w].push_back(out[j]);
+        }
+      });
+    for (auto& t : th) t.join();
+    stop = true;
+    producer.join();
+    std::vector<uint8_t> seen(g_head.load(), 0);
+    uint64_t total = 0;
+    for (auto& v : got)
+      for (uint64_t t : v) {
+        ++total;
+        if (t >= seen.size()) fail("ticket at or beyond Head");
+        else if (seen[t]++) fail("ticket handed out twice");
+      }
+    if (total != g_head.load()) fail("tickets below Head not all handed out");
+    empties += trial_empties;
+    groups_with_items += total ? 1 : 0;
+  }
+  std::printf("trials %d empties %llu failures %llu\n", trials, (unsigned long long)empties,
+              (unsigned long long)g_failures.load());
+  return g_failures.load() ? 1 : 0;
+}

@@ -1,0 +1,35 @@
+"""CPU model of the workgroup claim-combining protocol (combine_claim in
+gcq_amd/csrc/gcq/prim.hpp): tickets unique and dense, every group member's
+call before the leader's Tail read, no deadlock, across 2-16 waves.  The same
+model with the Tail read moved before the group's close (a planted ordering
+mistake) must be caught.  CPU only."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "combine_model.cpp")
+
+
+def _build(tmp_path, name, *defs):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path / name)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", *defs, SRC, "-o", exe], check=True)
+    return exe
+
+
+def test_combine_protocol_model(tmp_path):
+    exe = _build(tmp_path, "combine_model")
+    r = subprocess.run([exe, "45", "3000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "failures 0" in r.stdout
+
+
+def test_combine_model_catches_late_close(tmp_path):
+    exe = _build(tmp_path, "combine_model_bad", "-DPLANT_LATE_CLOSE")
+    r = subprocess.run([exe, "30", "2000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "a member started after the leader read Tail" in r.stderr

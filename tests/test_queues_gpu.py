@@ -77,15 +77,33 @@ def test_fifo_check_concurrent_65536(gcq, kind):
 
 
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
-def test_throughput_conserves_items(gcq, kind):
-    print(f"[tb] kind={NAMES[kind]}", flush=True)
+@pytest.mark.parametrize("solo", [False, True], ids=["combined", "solo"])
+def test_throughput_conserves_items(gcq, kind, solo):
+    print(f"[tb] kind={NAMES[kind]} solo={solo}", flush=True)
     with _queue(gcq, kind, 4096, capacity=1 << 16) as q:
-        r = q.tb_run(1, threads=4096, chunk_ops=32, producer_percent=25, warmup_ms=20, run_ms=100)
+        r = q.tb_run(1, threads=4096, chunk_ops=32, producer_percent=25, warmup_ms=20, run_ms=100, solo_claims=solo)
     assert r["launches"] > 0 and r["elapsed_ms"] > 0
     assert r["enq_success"] > 0 and r["deq_success"] > 0
     # items dequeued in the window can exceed those enqueued in it only by
     # what was queued before the window (at most the capacity)
     assert r["deq_success"] <= r["enq_success"] + (1 << 16) + 4096 * 32
+
+
+@pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
+@pytest.mark.parametrize("pp", [25, 75])
+def test_combined_claims_split_histories(gcq, orc, kind, pp):
+    # 8,192 threads in blocks of 256 (4 waves), split roles: idle consumer
+    # waves combine their claims; every EMPTY they return must still be a
+    # linearizable answer for each wave of the group.
+    print(f"[hist combined] kind={NAMES[kind]} pp={pp}", flush=True)
+    with _queue(gcq, kind, 8192, capacity=8192) as q:
+        h = q.hist_run(threads=8192, ops=8, block=256, mode=1, producer_percent=pp)
+    assert len(h) == 8192 * 8
+    deq = h[h["op"] == 1]
+    assert np.any(deq["ret"] != 0)
+    if pp == 25:
+        assert np.any(deq["ret"] == 0)  # idle consumers answered EMPTY
+    _check(orc, h, f"{NAMES[kind]} combined split{pp}")
 
 
 def test_gwfq_slow_path_history(gcq, orc):
