@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--cpu-baseline-ms", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the BFS / batched / served / RT lines")
+    ap.add_argument("--no-solo", action="store_true",
+                    help="skip the solo-claims comparison run (profiles of the headline kernel alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -103,13 +105,30 @@ def main():
     peaks["striped_gtickets_per_s"]["4"] = round(p4["tickets_per_s"] / 1e9, 3)
     raw = g.faa_peak(threads=T, block=256, counters=1, stride_bytes=128, aggregate=False, iters=4, launches=3)
     peaks["per_lane_1_counter_gtickets_per_s"] = round(raw["tickets_per_s"] / 1e9, 4)
+    # One hot line shared by FAAs and polling loads, as the Tail line is in
+    # the split run: 25% of the waves add, 75% only load.  Its total access
+    # rate bounds a design in which every consumer attempt reads the line.
+    mix = g.faa_peak(threads=T, block=256, counters=1, stride_bytes=128, aggregate=True, iters=64, launches=10,
+                     poll_percent=75)
+    other = g.faa_peak(threads=T, block=256, counters=1, stride_bytes=128, aggregate=True, iters=64, launches=10,
+                       poll_percent=75, poll_other_line=True)
+    peaks["line_faa_plus_polling"] = {
+        "faa_per_s_M": round(mix["atomics_per_s"] / 1e6, 2), "loads_per_s_M": round(mix["loads_per_s"] / 1e6, 2),
+        "accesses_per_s_M": round((mix["atomics_per_s"] + mix["loads_per_s"]) / 1e6, 2),
+        "faa_per_s_M_pollers_on_other_line": round(other["atomics_per_s"] / 1e6, 2),
+        "faa_per_s_M_alone": round(p1["atomics_per_s"] / 1e6, 2)}
 
     # Consumers poll without sleeping between empty attempts (poll_backoff 0):
-    # measured fastest for this split (2129 Mops/s against 2001 and 1936 at
-    # backoff 1 and 2).
+    # measured fastest for this split (backoff 1 and 2 are 6% and 9% slower
+    # without combining, equal within 1% with it).  Idle consumer waves of a
+    # workgroup combine their claims (the library's default); the same run
+    # with solo claims is reported beside it.
     with g.Queue(kind, capacity=1 << 20, max_threads=max(T, 64), poll_backoff=0) as q:
         r = q.tb_run(g.SPLIT, threads=T, block=256, chunk_ops=128, producer_percent=25,
                      warmup_launches=args.warmup, timed_launches=args.steps)
+        solo = None if args.no_solo else q.tb_run(g.SPLIT, threads=T, block=256, chunk_ops=128, producer_percent=25,
+                                                   warmup_launches=args.warmup, timed_launches=args.steps,
+                                                   solo_claims=True)
     succ = r["enq_success"] + r["deq_success"]
     value = r["succ_mops"]
     ms_per_step = r["elapsed_ms"] / args.steps
@@ -133,7 +152,8 @@ def main():
         "config": {"workload": f"split-role throughput, {g.KIND_NAMES[kind]}, {T} threads, block 256, "
                                "25% producers, 128 ops/thread/launch, capacity 2^20",
                    "queue": args.queue, "threads": T, "block": 256, "chunk_ops": 128, "producer_percent": 25,
-                   "capacity": 1 << 20, "poll_backoff": 0, "parallelism": "single GPU"},
+                   "capacity": 1 << 20, "poll_backoff": 0, "claims": "combined per workgroup when idle",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "atomic", "achieved": round(achieved_ops / 1e9, 4),
                      "peak": round(p1["tickets_per_s"] / 1e9, 4), "unit": "Gop/s",
                      "frac": round(achieved_ops / p1["tickets_per_s"], 4),
@@ -147,6 +167,8 @@ def main():
         "atomic_peaks": peaks,
         "detail": {"enq_success": r["enq_success"], "deq_success": r["deq_success"], "deq_empty": r["deq_empty"],
                    "enq_fail": r["enq_fail"], "kernel_ms_mean": round(kernel_ms, 4),
+                   "solo_claims_mops": round(solo["succ_mops"], 3) if solo else None,
+                   "solo_claims_kernel_ms_mean": round(solo["kernel_ms_mean"], 4) if solo else None,
                    "elapsed_ms": round(r["elapsed_ms"], 3), "device": info["name"], "cus": info["cus"]},
     }
     if not args.no_cpu_baseline:

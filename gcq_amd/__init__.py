@@ -42,7 +42,8 @@ class FaaConfig(C.Structure):
 
 
 class FaaResult(C.Structure):
-    _fields_ = [("tickets_per_s", C.c_double), ("atomics_per_s", C.c_double), ("kernel_ms", C.c_double)]
+    _fields_ = [("tickets_per_s", C.c_double), ("atomics_per_s", C.c_double), ("kernel_ms", C.c_double),
+                ("loads_per_s", C.c_double)]
 
 
 class TbConfig(C.Structure):

@@ -309,6 +309,8 @@ extern "C" int gcq_faa_peak(const gcq_faa_config* cfg, gcq_faa_result* res) {
   res->kernel_ms = ms / cfg->launches;
   res->tickets_per_s = tickets / (ms * 1e-3);
   res->atomics_per_s = cfg->aggregate ? res->tickets_per_s / 64.0 : res->tickets_per_s;
+  const uint64_t poll_waves = (uint64_t)grid * cfg->block / 64 - faa_waves;
+  res->loads_per_s = (double)poll_waves * cfg->iters * cfg->launches / (ms * 1e-3);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   GCQ_HIP(hipFree(d));

@@ -90,6 +90,7 @@ typedef struct gcq_faa_result {
   double tickets_per_s;   /* counter increments per second (all lanes) */
   double atomics_per_s;   /* device atomic instructions per second */
   double kernel_ms;       /* mean launch duration */
+  double loads_per_s;     /* pollers' load instructions per second (one per wave) */
 } gcq_faa_result;
 /* Atomic roofline microbenchmark; no reference counterpart (the reference
  * measures no peak).  Denominator of the split-role roofline. */
