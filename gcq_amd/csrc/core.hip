@@ -397,7 +397,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
   __shared__ gcq::Combine s_cb;
   if (threadIdx.x == 0) gcq::combine_init(&s_cb);
   __syncthreads();
-  gcq::Combine* const cb = solo ? nullptr : &s_cb;
+  gcq::Combine* const cb = solo || gridDim.x * blockDim.x < gcq::kCombineMinThreads ? nullptr : &s_cb;
   uint64_t eo = 0, ef = 0, dok = 0, de = 0;
   if (tid < threads) {
     const uint64_t seq0 = (uint64_t)launch * chunk + 1;

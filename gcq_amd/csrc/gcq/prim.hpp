@@ -215,6 +215,12 @@ __device__ __forceinline__ void lgkm_wait() { asm volatile("s_waitcnt lgkmcnt(0)
 // claim alone, so a busy queue does not wait on group turns.  The per-wave
 // count is only a choice of path; its LDS contents at launch do not matter.
 constexpr uint32_t kCombineAfterEmpty = 4;
+// Smallest grid worth combining for.  With fewer polling waves the hot lines
+// are not congested and a group's turn only adds latency.  Measured, split
+// 25/75, block 256, combined over solo: G-WFQ 0.87 / 1.00 / 1.03 / 1.04 /
+// 1.08 at 2,048 / 4,096 / 8,192 / 16,384 / 65,536 threads; G-LFQ 0.97 /
+// 0.99 / 1.03 / 1.05 / 1.05.
+constexpr uint32_t kCombineMinThreads = 8192;
 __device__ __forceinline__ uint8_t* combine_recent() {
   __shared__ uint8_t s_recent[16];
   return &s_recent[wave_in_block()];

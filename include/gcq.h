@@ -117,7 +117,8 @@ typedef struct gcq_tb_config {
   uint32_t warmup_launches;   /* > 0 with timed_launches: launch-count mode instead */
   uint32_t timed_launches;    /* exactly this many timed launches (0 = time mode) */
   uint32_t solo_claims;       /* 1 = no workgroup combining of idle consumers'
-                                 claims (G-WFQ, G-LFQ); 0 (default) = combine */
+                                 claims (G-WFQ, G-LFQ); 0 (default) = combine
+                                 in grids of 8,192 threads or more */
 } gcq_tb_config;
 typedef struct gcq_tb_result {
   uint64_t enq_success, enq_fail, deq_success, deq_empty;

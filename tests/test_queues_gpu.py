@@ -79,14 +79,15 @@ def test_fifo_check_concurrent_65536(gcq, kind):
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
 @pytest.mark.parametrize("solo", [False, True], ids=["combined", "solo"])
 def test_throughput_conserves_items(gcq, kind, solo):
+    # 8,192 threads: the smallest grid in which the harness combines claims
     print(f"[tb] kind={NAMES[kind]} solo={solo}", flush=True)
-    with _queue(gcq, kind, 4096, capacity=1 << 16) as q:
-        r = q.tb_run(1, threads=4096, chunk_ops=32, producer_percent=25, warmup_ms=20, run_ms=100, solo_claims=solo)
+    with _queue(gcq, kind, 8192, capacity=1 << 16) as q:
+        r = q.tb_run(1, threads=8192, chunk_ops=32, producer_percent=25, warmup_ms=20, run_ms=100, solo_claims=solo)
     assert r["launches"] > 0 and r["elapsed_ms"] > 0
     assert r["enq_success"] > 0 and r["deq_success"] > 0
     # items dequeued in the window can exceed those enqueued in it only by
     # what was queued before the window (at most the capacity)
-    assert r["deq_success"] <= r["enq_success"] + (1 << 16) + 4096 * 32
+    assert r["deq_success"] <= r["enq_success"] + (1 << 16) + 8192 * 32
 
 
 @pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
