@@ -20,11 +20,44 @@ using gcq::lane_id;
 constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
 
 // ---------------------------------------------------------------- adapters
+// Counted pops (kCounted): in a level-synchronous traversal a level's input
+// ring holds exactly the tickets [Head, Tail) at the level's start, all of
+// them written before the previous barrier.  A pop is then one aggregated FAA
+// on Head compared with that end: no Head/Tail claim reads, and the level's
+// last pop costs one FAA.  Head overshoots the end by up to a wave's lanes per
+// wave; drained() at the barrier sets it back to the end.
 struct FGwf {
   static constexpr int kQueues = 2;
+  static constexpr bool kCounted = true;
   gcq::gwf::Ring* r;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
+  // Head and Tail words pack (count << thr_bits | thread index); the index
+  // field is the null index whenever no slow-path request is being
+  // published, as at a level's start and end.
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::gwf::tail(r)) >> r->g.thr_bits; }
+  __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
+    using namespace gcq::gwf;
+    NoProbe np;
+    RingOps<> o(r, tid, &np);
+    const uint64_t T = o.cnt_of(gcq::wave_faa(head(r), (uint64_t)1 << r->g.thr_bits));
+    if (T >= end) return -1;
+    const uint32_t j = o.slot(T), tcyc = o.cyc(T);
+    uint64_t E = gcq::ld_rlx(&slots(r)[j]);
+    for (uint32_t w = 0; (e_cyc(E) != tcyc || e_idx(E) >= kBot) && w < gcq::kSlotWaitSpins; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      E = gcq::ld_rlx(&slots(r)[j]);
+    }
+    if (e_cyc(E) != tcyc || e_idx(E) >= kBot) return 0;
+    o.consume(T, j, E);  // also finalises an item a slow-path enqueue left
+    *v = e_idx(E);
+    return 1;
+  }
+  __device__ __forceinline__ void drained() const {
+    const uint32_t B = r->g.thr_bits;
+    const uint64_t t = gcq::ld_rlx(gcq::gwf::tail(r)) >> B;
+    gcq::st_rlx(gcq::gwf::head(r), (t << B) | ((1ull << B) - 1ull));
+  }
   __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
     gcq::gwf::NoProbe np;
     gcq::gwf::RingOps<> o(r, tid, &np);
@@ -42,9 +75,37 @@ struct FGwf {
 };
 struct FLfq {
   static constexpr int kQueues = 2;
+  static constexpr bool kCounted = true;
   gcq::lfq::Ring* r;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
+  // End of the level's tickets: the input ring's Tail, which no one moves
+  // during the level.
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::lfq::tail(r)); }
+  // 1 = *v popped, 0 = this ticket holds no item (skip it), -1 = the level
+  // is drained.
+  __device__ __forceinline__ int pop_counted(uint32_t, uint64_t end, uint32_t* v) const {
+    using namespace gcq::lfq;
+    const uint64_t T = gcq::wave_faa(head(r), (uint64_t)1);
+    if (T >= end) return -1;
+    const Geometry g = r->g;
+    const uint32_t tcyc = cycle_of(T, g.slots_log2);
+    uint64_t* const e = slots(r) + slot_of(T, g.slots_log2, g.remap_log2);
+    uint64_t E = gcq::ld_rlx(e);
+    // Written before the barrier; the wait only guards against a ticket an
+    // enqueuer abandoned.
+    for (uint32_t w = 0; (e_cyc(E) != tcyc || e_idx(E) >= kBot) && w < gcq::kSlotWaitSpins; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      E = gcq::ld_rlx(e);
+    }
+    if (e_cyc(E) != tcyc || e_idx(E) >= kBot) return 0;
+    gcq::or_rlx(e, (uint64_t)kBotC);  // consume: index := ⊥c, keep cycle/safe
+    *v = e_idx(E);
+    return 1;
+  }
+  __device__ __forceinline__ void drained() const {
+    gcq::st_rlx(gcq::lfq::head(r), gcq::ld_rlx(gcq::lfq::tail(r)));
+  }
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     gcq::lfq::enqueue(r, v);
     return true;
@@ -58,6 +119,7 @@ struct FLfq {
 };
 struct FSfq {
   static constexpr int kQueues = 2;
+  static constexpr bool kCounted = false;
   gcq::sfq::Queue* q;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
@@ -77,6 +139,7 @@ struct FSfq {
 // is cleared by the grid and rewound at the level's barrier.
 struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt.hpp:42-46)
   static constexpr int kQueues = 3;
+  static constexpr bool kCounted = false;
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
   __device__ __forceinline__ void recycle(uint64_t gid, uint64_t stride, uint32_t tid) const {
@@ -162,6 +225,64 @@ __device__ bool grid_barrier(Barrier* b, uint32_t nblocks, Last last) {
   }
   __syncthreads();
   return gcq::ld_rlx(&b->abort) == 0u;
+}
+
+// Level barrier of the persistent BFS: arrival and the level's discovery
+// count travel in one 64-bit FAA per block ([63:32] blocks arrived, [31:0]
+// vertices discovered), and the release word carries the next level's size
+// ([63:32] level + 1, [31:0] count).  One atomic and a spin per level and
+// block, instead of a counter FAA per wave, a generation read, the arrival
+// FAA and a count read.
+struct LevelBarrier {
+  uint64_t arrive, p0[15];
+  uint64_t release, p1[15];
+  uint32_t abort, p2[31];
+};
+
+// All threads of the block call it; wave_disc is the calling wave's
+// discovery count (wave-uniform).  *next gets the next level's size.  The last
+// block to arrive runs last() first.  Returns false once any block has
+// aborted (spin limit).
+template <class Last>
+__device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint32_t wave_disc, uint32_t* next,
+                              Last last) {
+  __shared__ uint32_t s_wd[16];
+  __shared__ uint32_t s_next, s_ok;
+  gcq::vm_wait();  // this wave's pushes have completed
+  if (lane_id() == 0) s_wd[(threadIdx.x / 64) & 15] = wave_disc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < (blockDim.x + 63) / 64; ++k) sum += s_wd[k];
+    const uint64_t old = gcq::faa_rlx(&b->arrive, (1ull << 32) | sum);
+    uint32_t ok = 1, n = 0;
+    if ((uint32_t)(old >> 32) == nblocks - 1) {
+      n = (uint32_t)old + sum;
+      last();
+      gcq::st_rlx(&b->arrive, 0ull);
+      gcq::vm_wait();  // reset (and last()) complete before the release
+      gcq::st_rlx(&b->release, ((uint64_t)(uint32_t)(L + 1) << 32) | n);
+    } else {
+      for (uint64_t spins = 0;; ++spins) {
+        const uint64_t r = gcq::ld_rlx(&b->release);
+        if ((uint32_t)(r >> 32) == (uint32_t)(L + 1)) {
+          n = (uint32_t)r;
+          break;
+        }
+        if (gcq::ld_rlx(&b->abort) || spins > kBarrierSpinLimit) {
+          gcq::st_rlx(&b->abort, 1u);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    s_next = n;
+    s_ok = ok;
+  }
+  __syncthreads();
+  *next = s_next;
+  return s_ok != 0;
 }
 
 // ---------------------------------------------------------------- reset kernels
