@@ -61,6 +61,15 @@ struct LevelAcc {
 // kCounted: pop with F::pop_counted against the level's end (persistent
 // kernel only: the end holds because every push of the level before was
 // complete at its barrier, and the barrier sets Head back afterwards).
+// Each neighbour's depth is tested with a cached load before its CAS when
+// the wave's batch has more than kProbeEdges edges.  The probe saves an
+// atomic per visited neighbour, which pays on long adjacency scans.  It also
+// adds a dependent round trip to each 64-edge round, which dominates a batch
+// of a few rounds.  Measured: the 4900^2 lattice (about 3 edges per vertex)
+// runs 10% faster without the probe; R-MAT scale 20 at 65,536 threads runs
+// 18% slower without it.
+constexpr int32_t kProbeEdges = 256;
+
 template <bool kCounted, class F>
 __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restrict__ offs,
                              const int32_t* __restrict__ idx, int32_t* level, int32_t L, uint32_t tid, uint32_t* stage,
@@ -93,6 +102,7 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
     }
     const int32_t excl = incl - deg;
     const int32_t total = __shfl(incl, 63, 64);
+    const bool probe = total > kProbeEdges;
     for (int32_t base = 0; base < total; base += 64) {
       const int32_t e = base + (int32_t)lane;
       const int32_t ec = e < total ? e : total - 1;
@@ -111,7 +121,7 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
         w = (uint32_t)idx[obeg + (ec - oexcl)];
         // A depth changes once per traversal, from -1 to a level, so a cached
         // (possibly stale) read can only send a visited vertex to the CAS.
-        if (gcq::ld_cached(&level[w]) == -1) {
+        if (!probe || gcq::ld_cached(&level[w]) == -1) {
           int32_t expect = -1;
           disc = gcq::cas_rlx(&level[w], expect, L + 1);
         }
