@@ -41,9 +41,9 @@ namespace bfsk {
 
 using namespace gcqf;
 
-// Control block: the barriers, then each contended counter on its own 128-B line.
+// Control block: the persistent kernel's level barrier, then each contended
+// counter on its own 128-B line (next[] counts the host-loop levels).
 struct Ctl {
-  Barrier bar;
   LevelBarrier lbar;
   uint32_t next[3], p2[29];
   uint32_t err, p3[31];
@@ -432,7 +432,7 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   res->levels = h.levels;
   res->kernel_ms = kms;
   res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  if (h.bar.abort || h.lbar.abort) {
+  if (h.lbar.abort) {
     gcqh::set_error("BFS aborted: a grid-barrier wait exceeded its spin limit");
     return GCQ_ETIMEOUT;
   }
