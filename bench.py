@@ -61,13 +61,16 @@ def _profile_traffic(name):
         return None
 
 
-def cpu_baseline(kind, threads, run_ms):
+QUEUE_NAMES = {0: "G-WFQ", 1: "G-LFQ", 2: "WFQ", 3: "SFQ"}
+
+
+def cpu_baseline(kind, threads, run_ms, warmup_ms=200):
     from oracle import pyoracle as orc
-    r = orc.tb_run(kind, threads=threads, mode=1, producer_percent=25, chunk_ops=128, warmup_ms=200,
+    r = orc.tb_run(kind, threads=threads, mode=1, producer_percent=25, chunk_ops=128, warmup_ms=warmup_ms,
                    run_ms=run_ms, capacity=1 << 20)
     return {"value": round(r["succ_mops"], 3), "unit": "Mops/s", "cores": threads, "kind": "port",
-            "sample": f"host std::atomic G-WFQ restatement, split 25/75, {threads} threads, chunk 128, "
-                      f"200 ms warmup + {run_ms} ms timed, capacity 2^20 (hardware_concurrency="
+            "sample": f"host std::atomic {QUEUE_NAMES[kind]} restatement, split 25/75, {threads} threads, chunk 128, "
+                      f"{warmup_ms} ms warmup + {run_ms} ms timed, capacity 2^20 (hardware_concurrency="
                       f"{r['hw_threads']})"}
 
 
@@ -200,7 +203,7 @@ def secondary(g, with_cpu):
         rows = {}
         extra = {"wfq_segment_cells": 1 << 28} if kind == g.WFQ else {}
         for mode, mname in ((g.SPLIT, "split25"), (0, "balanced")):
-            for T in (1024, 4096, 16384, 65536):
+            for T in (1024, 2048, 4096, 8192, 16384, 32768, 65536):
                 with g.Queue(kind, capacity=1 << 20, max_threads=65536, poll_backoff=0, **extra) as q:
                     r = q.tb_run(mode, threads=T, block=256, chunk_ops=128, producer_percent=25,
                                  warmup_launches=5, timed_launches=20)
@@ -208,6 +211,10 @@ def secondary(g, with_cpu):
         with g.Queue(kind, capacity=1 << 20, max_threads=65536, **extra) as q:
             f = q.fifo_check(65536, 32, concurrent=True)
         rows["fifo_check_65536x32_verified"] = bool(f["verified"])
+        if with_cpu:
+            # the host share of one GPU is 16 cores whatever os.cpu_count() reports
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            rows["cpu_baseline"] = cpu_baseline(kind, cores, 300, warmup_ms=50)
         sweep[names[kind]] = rows
     out["queue_sweep"] = {"unit": "Mops/s (successful enq + deq)", "chunk_ops": 128, "block": 256,
                           "launches": "5 warmup + 20 timed", "capacity": 1 << 20, "queues": sweep}
