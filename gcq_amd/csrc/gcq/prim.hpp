@@ -188,8 +188,9 @@ __device__ __forceinline__ T wave_faa(T* p, nd_t<T> step) {
 //    hands (k, base) to each member's LDS slot.  Lane r of a member at offset
 //    o gets ticket base + o + r when o + r < k.
 // All waits are on LDS, between waves of one workgroup, which are all
-// resident.  The leader of group g waits only for the leader of group g - 1,
-// which has already closed g - 1 and has no further waits.  A kernel that
+// resident.  The leader of group g waits (boundedly) only for the leader of
+// group g - 1, which has already closed g - 1; a member waits only for its
+// group's leader.  A kernel that
 // passes a Combine to a dequeue calls combine_init() from one thread, then
 // a barrier, before any dequeue.
 struct Combine {
@@ -226,6 +227,8 @@ __device__ __forceinline__ uint8_t* combine_recent() {
   return &s_recent[wave_in_block()];
 }
 
+constexpr uint32_t kCombineLeaderSpins = 1u << 20;
+
 // Run by the calling wave's leader lane only.  `memclaim(lanes, &k, &base)`
 // runs in the group leader after the close: it reads Head, then Tail, and
 // takes k tickets starting at base.  Returns this wave's offset in the group;
@@ -239,8 +242,13 @@ __device__ __forceinline__ uint32_t combine_claim(Combine* c, uint32_t lanes, ui
   const uint32_t grp = (uint32_t)(old >> 48);
   const uint32_t off = (uint32_t)old;
   if (off == 0) {
-    while ((__hip_atomic_load(&c->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xFFFFu) !=
-           ((grp - 1u) & 0xFFFFu))
+    // The wait only lets more waves join; the answer's validity rests on the
+    // close below.  Bounding it means every leader reaches its publish, so no
+    // member waits on a leader that is itself waiting.
+    for (uint32_t s = 0; s < kCombineLeaderSpins &&
+                         (__hip_atomic_load(&c->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xFFFFu) !=
+                             ((grp - 1u) & 0xFFFFu);
+         ++s)
       __builtin_amdgcn_s_sleep(1);
     const uint64_t fin = __hip_atomic_exchange(&c->join, (uint64_t)((grp + 1u) & 0xFFFFu) << 48, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
