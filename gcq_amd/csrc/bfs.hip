@@ -25,8 +25,8 @@
 //    levels).  Every block is co-resident (checked on the host) and every
 //    barrier wait has a spin limit that aborts the run instead of hanging.
 //    Its barrier carries arrival and the level's discovery count in one
-//    64-bit FAA per block, and G-WFQ / G-LFQ frontiers are drained by
-//    counted pops (frontier.hpp): 12.2 us per lattice level instead of 21.
+//    64-bit FAA per block, and every frontier kind is drained by counted
+//    pops (frontier.hpp): 12.2 us per lattice level instead of 21.
 #include <algorithm>
 #include <chrono>
 #include <cstring>

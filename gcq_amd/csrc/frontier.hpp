@@ -119,10 +119,35 @@ struct FLfq {
 };
 struct FSfq {
   static constexpr int kQueues = 2;
-  static constexpr bool kCounted = false;
+  static constexpr bool kCounted = true;
   gcq::sfq::Queue* q;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
+  // Counted pops: the slot of a ticket below the level's end carries its
+  // item and the "full" phase id; the pop reads the item and hands the slot
+  // to the next lap, as a blocking dequeue does after its wait.
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::sfq::tail(q)); }
+  __device__ __forceinline__ int pop_counted(uint32_t, uint64_t end, uint32_t* v) const {
+    using namespace gcq::sfq;
+    const uint64_t T = gcq::wave_faa(head(q), (uint64_t)1);
+    if (T >= end) return -1;
+    Slot* s = &slots(q)[T & (q->g.qlen - 1)];
+    const uint64_t id = 2ull * (T >> q->g.qlen_log2) + 1ull;
+    bool ready = gcq::ld_rlx(&s->id) == id;
+    for (uint32_t w = 0; !ready && w < gcq::kSlotWaitSpins; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      ready = gcq::ld_rlx(&s->id) == id;
+    }
+    if (!ready) return 0;
+    const uint64_t x = gcq::ld_rlx(&s->item);
+    gcq::vm_wait();  // item read completes before the slot is handed on
+    gcq::st_rlx(&s->id, id + 1);
+    *v = (uint32_t)x;
+    return 1;
+  }
+  __device__ __forceinline__ void drained() const {
+    gcq::st_rlx(gcq::sfq::head(q), gcq::ld_rlx(gcq::sfq::tail(q)));
+  }
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     return gcq::sfq::enqueue_blocking(q, (uint64_t)v, 1ull << 22) == gcq::sfq::kSuccess;
   }
@@ -139,9 +164,32 @@ struct FSfq {
 // is cleared by the grid and rewound at the level's barrier.
 struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt.hpp:42-46)
   static constexpr int kQueues = 3;
-  static constexpr bool kCounted = false;
+  static constexpr bool kCounted = true;
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
+  // Counted pops: a level's input queue was only enqueued into during the
+  // level before, so every cell in [H, T) holds its value (an enqueue's CAS
+  // fails only on a cell a dequeuer marked).  A pop reads the value; no
+  // dequeue request is pending, so no helper competes for the cell.  The
+  // queue is recycled and rewound as the idle frontier a level later, which
+  // also clears Head's overshoot.
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::ymc::Tp(q)); }
+  __device__ __forceinline__ int pop_counted(uint32_t, uint64_t end, uint32_t* v) const {
+    using namespace gcq::ymc;
+    const uint64_t T = gcq::wave_faa(Hp(q), (uint64_t)1);
+    if (T >= end) return -1;
+    if (T >= q->g.cells) return 0;
+    Cell* c = &cells(q)[T];
+    uint64_t x = gcq::ld_rlx(&c->val);
+    for (uint32_t w = 0; x == kBottom && w < gcq::kSlotWaitSpins; ++w) {
+      __builtin_amdgcn_s_sleep(1);
+      x = gcq::ld_rlx(&c->val);
+    }
+    if (x == kBottom || x == kTop || x == kRetry) return 0;
+    *v = (uint32_t)(x - 1);
+    return 1;
+  }
+  __device__ __forceinline__ void drained() const {}
   __device__ __forceinline__ void recycle(uint64_t gid, uint64_t stride, uint32_t tid) const {
     gcq::ymc::recycle_cells(q, gid, stride);
     gcq::ymc::reset_handle(q, h, tid);

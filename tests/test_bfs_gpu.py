@@ -98,7 +98,7 @@ def test_wfq_cells_recycled_65536_threads(gcq, orc, mode):
         _run_and_check(orc, g, 0, 2, 65536, mode)
 
 
-@pytest.mark.parametrize("kind", [1, 0], ids=["glfq", "gwfq"])
+@pytest.mark.parametrize("kind", [1, 0, 2, 3], ids=["glfq", "gwfq", "wfq", "sfq"])
 def test_config3_lattice_4900(gcq, orc, kind):
     # The benchmark graph: 4900 x 4900 lattice, p = 0.75, 1% diagonals,
     # seed 7, source 0, 8,192 threads (block 256), persistent: 9,371 levels
