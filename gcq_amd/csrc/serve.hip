@@ -56,7 +56,7 @@ struct alignas(64) Req {
   uint64_t id;   // server-internal request number
   int32_t type, graph, src, slot;
   int32_t x0, y0, w, h;
-  uint64_t buf, buf2;  // BFS: depth array; RT: image, ray records (device arena)
+  uint64_t buf, buf2;  // BFS: depth array, visited bitmap; RT: image, ray records (device arena)
 };
 static_assert(sizeof(Req) == 64, "req");
 
@@ -152,6 +152,7 @@ __device__ void admit(const Dev& d, int32_t L) {
       gcq::st_rlx(&s->off, g.off);
       gcq::st_rlx(&s->idx, g.idx);
       gcq::st_rlx(&s->depth, buf);
+      gcq::st_rlx(&s->visited, gcq::ld_sys(&e->buf2));
       gcq::st_rlx(&s->src, (uint32_t)gcq::ld_sys(&e->src));
       gcq::st_rlx(&s->nv, g.nv);
       gcq::st_rlx(&s->start, (uint32_t)(L + 2));  // cleared during L+1, source expanded at L+2
@@ -213,22 +214,32 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) 
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
     // Queries installed at the previous barrier: clear their depth arrays
-    // (0 at the source) and put their sources into this level's output.
+    // (0 at the source) and visited bitmaps, and put their sources into this
+    // level's output.  Their first expansion is a level later, after the
+    // barrier's cache drop below.
     const uint32_t na = gcq::ld_rlx(&d.c->adm_n);
     for (uint32_t a = 0; a < na; ++a) {
       const uint32_t s = gcq::ld_rlx(&d.c->adm_slot[a]);
       const bk::Slot* sl = &d.slots[s];
       int32_t* dep = (int32_t*)gcq::ld_rlx(&sl->depth);
+      uint32_t* vis = (uint32_t*)gcq::ld_rlx(&sl->visited);
       const uint32_t nv = gcq::ld_rlx(&sl->nv), src = gcq::ld_rlx(&sl->src);
       for (uint64_t v = tid; v < nv; v += nthreads) gcq::st_rlx(&dep[v], v == src ? 0 : -1);
+      for (uint64_t w = tid; w < ((uint64_t)nv + 31) / 32; w += nthreads)
+        gcq::st_rlx(&vis[w], w == src / 32 ? 1u << (src & 31u) : 0u);
       if (tid == 0 && !fq.out(L).push(tid, (s << d.bd.vbits) | src)) gcq::or_rlx(&d.c->err, 1u);
     }
-    (void)expand(fq.in(L), fq.out(L), d.bd, L, tid, stage);
+    // Slot fields, depths and bitmaps are read through the caches: they only
+    // change at a barrier (admission) or, for a query not yet expanding, in
+    // the level after it, and every block drops its CU's and XCD's cached
+    // lines after each barrier.
+    (void)expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage);
     if (!grid_barrier(&d.c->bar, d.bfs_blocks, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
           admit(d, L);
         }))
       break;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (gcq::ld_rlx(&d.c->exit)) break;
     if (blockIdx.x == 0) done_scan(d, L);
   }
@@ -461,14 +472,16 @@ void release_all(gcq_server* s) {
 }
 
 uint64_t rt_bytes(int32_t w, int32_t h) { return (uint64_t)w * h * (12ull + sizeof(rtk::RayRec)); }
+// A BFS query's visited bitmap: one bit per vertex, whole 128-B lines.
+uint64_t vis_bytes(uint64_t nv) { return (nv + 1023) / 1024 * 128; }
 
 int check_request(gcq_server* s, const gcq_request* r) {
   GCQ_REQUIRE(!r->keep || s->keep_base, "kept results need keep_bytes > 0");
   if (r->type == GCQ_REQ_BFS) {
     GCQ_REQUIRE(r->graph >= 0 && r->graph < (int32_t)s->graphs.size(), "bad graph %d", r->graph);
     const uint64_t nv = (uint64_t)s->graphs[r->graph]->host.nv;
-    GCQ_REQUIRE(4 * nv <= s->cfg.budget_bytes, "query needs %llu bytes, above the budget",
-                (unsigned long long)(4 * nv));
+    GCQ_REQUIRE(4 * nv + vis_bytes(nv) <= s->cfg.budget_bytes, "query needs %llu bytes, above the budget",
+                (unsigned long long)(4 * nv + vis_bytes(nv)));
     GCQ_REQUIRE(nv <= (1ull << s->cfg.frontier_log2), "graph larger than the frontier ring");
   } else if (r->type == GCQ_REQ_RT) {
     GCQ_REQUIRE(s->scene, "server has no scene");
@@ -502,7 +515,12 @@ int try_admit(gcq_server* s) {
         GCQ_REQUIRE(!r.keep, "keep_bytes exhausted by kept results");
         break;
       }
+      if (!s->arena.alloc(vis_bytes(nv), &f.off2)) {
+        a1.release(f.off1, 4ull * nv);
+        break;
+      }
       f.len1 = 4ull * nv;
+      f.len2 = vis_bytes(nv);
       f.nv = nv;
       f.slot = s->free_bfs.back();
       s->free_bfs.pop_back();
@@ -510,6 +528,7 @@ int try_admit(gcq_server* s) {
       e.graph = r.graph;
       e.src = (r.src >= 0 && r.src < (int32_t)nv) ? r.src : 0;  // bfs_bench.cpp:555
       e.buf = (uint64_t)((r.keep ? s->keep_base : s->arena_base) + f.off1);
+      e.buf2 = (uint64_t)(s->arena_base + f.off2);
     } else {
       if (s->free_rt.empty()) break;
       const uint64_t img = 12ull * r.w * r.h, rec = (uint64_t)sizeof(rtk::RayRec) * r.w * r.h;
@@ -570,8 +589,8 @@ void collect(gcq_server* s) {
       s->frontier_used -= f.nv;
     } else {
       s->free_rt.push_back(f.slot);
-      s->arena.release(f.off2, f.len2);
     }
+    s->arena.release(f.off2, f.len2);  // BFS bitmap or RT ray records
     if (f.r.keep) s->kept[f.r.id] = sv::Kept{f.r.type, f.off1, f.len1};
     else s->arena.release(f.off1, f.len1);  // kept results stay in the keep arena until destroy
     s->ready.push_back(c);

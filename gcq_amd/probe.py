@@ -6,6 +6,7 @@ under rocprofv3 (kernel traces, HBM byte counters):
     python3 -m gcq_amd.probe bfs --runs 3
     python3 -m gcq_amd.probe batch --runs 3
     python3 -m gcq_amd.probe rt --runs 3
+    python3 -m gcq_amd.probe serve --runs 1
 """
 import argparse
 import json
@@ -14,7 +15,7 @@ import sys
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="gcq_amd.probe")
-    ap.add_argument("what", choices=["copy", "tb", "bfs", "batch", "rt", "rtc"])
+    ap.add_argument("what", choices=["copy", "tb", "bfs", "batch", "rt", "rtc", "serve"])
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--launches", type=int, default=20)
@@ -56,6 +57,22 @@ def main(argv=None):
                 else:
                     _, res = s.render_compact(width=1920, height=1080, block=1024)
         out.update(res)
+    elif a.what == "serve":
+        # the served line's burst: 300 requests at once, 90% BFS on R-MAT
+        # scales 14..22, 10% 256x256 tiles of the 64-sphere 1080p frame
+        from gcq_amd import bfs, rt, serve
+        graphs = [bfs.Graph.rmat(sc, 16, 0.57, 0.19, 0.19, 11) for sc in range(14, 23)]
+        nvs = [gr.info()[0] for gr in graphs]
+        rows = serve.poisson_trace(300, 1e9, nvs, (1920, 1080), rt_fraction=0.1, seed=6,
+                                   graph_sources=lambda gi, i: graphs[gi].pick_source(11, i))
+        with rt.Scene.preset(rt.RANDOM, 64, 3) as scene:
+            for _ in range(a.runs):
+                with serve.Server(graphs, scene, (1920, 1080), bfs_threads=229376, block=512, rt_threads=32768,
+                                  frontier_log2=28) as srv:
+                    res = serve.latency_summary(srv.run_trace(rows))
+        out.update(res)
+        for gr in graphs:
+            gr.close()
     print(json.dumps(out), flush=True)
 
 
