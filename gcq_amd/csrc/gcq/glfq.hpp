@@ -179,8 +179,10 @@ struct NoProbe {
 };
 
 // Enqueue a slot index; the ring must have room (indirection guarantees it).
+// scq_reset: reset the threshold in SCQ's load-then-store form (the
+// free-index ring; reset_threshold).
 template <class Probe = NoProbe>
-__device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
+__device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool scq_reset = false) {
   const Geometry g = r->g;
   uint64_t* const e = slots(r);
   for (;;) {
@@ -204,13 +206,12 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
       ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));
     }
     // Threshold back to 3n - 1 after the insertions (glf_queue.hpp:493-495):
-    // one store per wave, issued after every successful lane's CAS has
-    // returned, instead of a load (and exchange) per lane on the critical
-    // path.  Storing 3n - 1 when it already holds 3n - 1 is harmless, and a
-    // reset that lands after a concurrent decrement is the same as a late
-    // reset.
+    // one reset per wave, issued after every successful lane's CAS has
+    // returned, instead of a load (and exchange) per lane.  Storing 3n - 1
+    // when it already holds 3n - 1 is harmless, and a reset that lands after
+    // a concurrent decrement is the same as a late reset.
     const uint64_t okm = __ballot(ok);
-    if (okm && lane_id() == first_lane(okm)) st_rlx(thr(r), g.thresh_max);
+    if (okm && lane_id() == first_lane(okm)) reset_threshold(thr(r), g.thresh_max, scq_reset);
     if (ok) return;
   }
 }
@@ -218,8 +219,9 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr) {
 // Dequeue a slot index, or kBot when the ring is empty.
 // idle_check: consumer side, tickets bounded by the items present.  The
 // free-index ring of the indirection (idle_check false) takes plain
-// aggregated tickets while it has indices and bounded ones after this wave
-// last found it empty, so producers of a full queue do not mark free-ring
+// aggregated tickets while it has indices and bounded ones for kLowStreak
+// dequeues after this wave finds it empty, so producers of a full queue do
+// not mark free-ring
 // slots ahead of the consumers returning indices.  The per-wave hint only
 // picks the ticket path; its initial LDS contents do not matter.
 template <class Probe = NoProbe>
@@ -231,10 +233,10 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
   bool bounded = idle_check;
   if (!idle_check) {
     if (ld_rlx(thr(r)) < 0) {  // consumer side: the claim tests emptiness
-      s_low[wv] = 1;
+      s_low[wv] = low_next(s_low[wv], true);
       return kBot;
     }
-    bounded = s_low[wv] != 0;
+    bounded = (s_low[wv] & 15u) != 0;
   }
   if (idle_check) idle_pause(g.poll_backoff);
   for (;;) {
@@ -251,7 +253,7 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
       }
       if (H == kEmptyTicket) {
         if (idle_check) idle_note(g.poll_backoff, true);
-        else s_low[wv] = 1;
+        else s_low[wv] = low_next(s_low[wv], true);
         return kBot;
       }
       if (H == kNoTicket) {
@@ -277,7 +279,7 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
       if (ecyc == hcyc && eidx < kBot) {
         or_rlx(&e[j], (uint64_t)kBotC);  // consume: index := ⊥c, keep cycle/safe
         if (idle_check) idle_note(g.poll_backoff, false);
-        else s_low[wv] = 0;
+        else s_low[wv] = low_next(s_low[wv], false);
         return eidx;
       }
       const uint64_t nv = (eidx >= kBot) ? e_pack(hcyc, e_safe(E), kBot) : e_pack(ecyc, false, eidx);
@@ -291,11 +293,11 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
     if (T <= H + 1) {
       catchup(r, H + 1);
       faa_rlx(thr(r), -1);
-      if (!idle_check) s_low[wv] = 1;
+      if (!idle_check) s_low[wv] = low_next(s_low[wv], true);
       return kBot;
     }
     if (faa_rlx(thr(r), -1) <= 0) {
-      if (!idle_check) s_low[wv] = 1;
+      if (!idle_check) s_low[wv] = low_next(s_low[wv], true);
       return kBot;
     }
   }
@@ -332,7 +334,7 @@ __device__ inline bool dequeue_ptr(const Mpmc& q, uint64_t* out, Probe* pr = nul
   const uint32_t idx = dequeue(q.aq, pr, true, cb);
   if (idx == kBot) return false;  // empty
   *out = ld_rlx(&q.data[idx]);  // address-dependent on the dequeued index;
-  enqueue(q.fq, idx, pr);        // completes before enqueue() touches a slot
+  enqueue(q.fq, idx, pr, true);  // completes before enqueue() touches a slot
   return true;
 }
 

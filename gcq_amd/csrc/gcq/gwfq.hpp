@@ -256,7 +256,9 @@ struct RingOps {
   }
 
   // Fast path enqueue attempt.  0 = done, else the raw ticket that failed.
-  __device__ __forceinline__ uint64_t try_enq(uint32_t idx) {
+  // scq_reset: reset the threshold in SCQ's load-then-store form (the
+  // free-index ring; reset_threshold).
+  __device__ __forceinline__ uint64_t try_enq(uint32_t idx, bool scq_reset = false) {
     const uint64_t T = cnt_of(wave_faa(tail(r), (uint64_t)1 << g.thr_bits));
     // Every earlier memory operation of this wave completes before the slot
     // is touched: this orders a payload store before its index is published
@@ -280,13 +282,12 @@ struct RingOps {
       ok = cas_rlx(&e[j], E, e_pack(fresh_note(e_note(E), tcyc), tcyc, true, true, 0, idx));
     }
     // Threshold back to 3n - 1 after the insertions (gwf_ring.hpp:409-410):
-    // one store per wave, issued after every successful lane's CAS has
-    // returned, instead of a load (and exchange) per lane on the critical
-    // path.  Storing 3n - 1 when it already holds 3n - 1 is harmless, and a
-    // reset that lands after a concurrent decrement is the same as a late
-    // reset.
+    // one reset per wave, issued after every successful lane's CAS has
+    // returned, instead of a load (and exchange) per lane.  Storing 3n - 1
+    // when it already holds 3n - 1 is harmless, and a reset that lands after
+    // a concurrent decrement is the same as a late reset.
     const uint64_t okm = __ballot(ok);
-    if (okm && lane_id() == first_lane(okm)) st_rlx(thr(r), g.thresh_max);
+    if (okm && lane_id() == first_lane(okm)) reset_threshold(thr(r), g.thresh_max, scq_reset);
     return ok ? 0 : T;
   }
 
@@ -601,11 +602,11 @@ struct RingOps {
   }
 
   // Enqueue_wf (gwf_ring.hpp:1234-1284).
-  __device__ void enqueue(uint32_t idx) {
+  __device__ void enqueue(uint32_t idx, bool scq_reset = false) {
     help_threads();
     uint64_t T = 0;
     for (uint32_t c = g.enq_patience; --c != 0;) {
-      T = try_enq(idx);
+      T = try_enq(idx, scq_reset);
       if (T == 0) return;
     }
     pr->slow_entry();
@@ -630,7 +631,8 @@ struct RingOps {
   // idle_check: consumer-side dequeue with tickets bounded by the items
   // present (claim_head).  The free-index ring of the indirection (idle_check
   // false) takes plain aggregated tickets while it has indices, and bounded
-  // ones after this wave last found it empty: when the queue runs full,
+  // ones for kLowStreak dequeues after this wave finds it empty: when the
+  // queue runs full,
   // producers taking plain tickets on an empty free ring would mark its slots
   // ahead of the consumers returning indices and send every one of those
   // round again (throughput fell from ~1,000 to ~6 Mops/s at 16,384 threads,
@@ -644,10 +646,10 @@ struct RingOps {
     bool bounded = idle_check;
     if (!idle_check) {
       if (ld_rlx(thr(r)) < 0) {
-        s_low[wv] = 1;
+        s_low[wv] = low_next(s_low[wv], true);
         return kBot;
       }
-      bounded = s_low[wv] != 0;
+      bounded = (s_low[wv] & 15u) != 0;
     }
     if (idle_check) idle_pause(g.poll_backoff);
     help_threads();
@@ -657,7 +659,7 @@ struct RingOps {
       H = try_deq(&out, bounded, cb);
       if (H == 0) {
         if (idle_check) idle_note(g.poll_backoff, out == kBot);
-        else s_low[wv] = out == kBot ? 1 : 0;
+        else s_low[wv] = low_next(s_low[wv], out == kBot);
         return out;
       }
       if (H != kNoTicket) last = H;
@@ -739,7 +741,7 @@ __device__ inline bool dequeue_ptr(const Mpmc& q, uint32_t tid, uint64_t* out, P
   const uint32_t idx = RingOps<Probe>(q.aq, tid, pp).dequeue(true, cb);
   if (idx == kBot) return false;  // empty
   *out = ld_rlx(&q.data[idx]);  // address-dependent; completes before try_enq touches a slot
-  RingOps<Probe>(q.fq, tid, pp).enqueue(idx);
+  RingOps<Probe>(q.fq, tid, pp).enqueue(idx, true);
   return true;
 }
 

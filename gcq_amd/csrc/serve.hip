@@ -233,7 +233,7 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) 
     // change at a barrier (admission) or, for a query not yet expanding, in
     // the level after it, and every block drops its CU's and XCD's cached
     // lines after each barrier.
-    (void)expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage);
+    (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage);
     if (!grid_barrier(&d.c->bar, d.bfs_blocks, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
           admit(d, L);
