@@ -204,8 +204,8 @@ class Queue:
                launches_per_sync=0, warmup_launches=0, timed_launches=0, solo_claims=False):
         """Timed balanced/split run: a time window (run_ms) or, with
         timed_launches > 0, exactly that many launches after warmup_launches.
-        solo_claims turns off the workgroup combining of idle consumers'
-        claims (G-WFQ, G-LFQ)."""
+        solo_claims turns off workgroup combining (G-WFQ, G-LFQ): of idle
+        consumers' claims and of producers' Tail tickets."""
         cfg = TbConfig(mode, threads, block, chunk_ops, producer_percent, warmup_ms, run_ms, launches_per_sync,
                        warmup_launches, timed_launches, int(bool(solo_claims)))
         res = TbResult()

@@ -394,10 +394,15 @@ template <class Q>
 __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32_t pp, uint32_t launch,
                      uint32_t solo, uint64_t* stats) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  __shared__ gcq::Combine s_cb;
-  if (threadIdx.x == 0) gcq::combine_init(&s_cb);
+  __shared__ gcq::Combine s_cb, s_ct;
+  if (threadIdx.x == 0) {
+    gcq::combine_init(&s_cb);
+    gcq::combine_init(&s_ct);
+  }
   __syncthreads();
-  gcq::Combine* const cb = solo || gridDim.x * blockDim.x < gcq::kCombineMinThreads ? nullptr : &s_cb;
+  const uint32_t grid_threads = gridDim.x * blockDim.x;
+  gcq::Combine* const cb = solo || grid_threads < gcq::kCombineMinThreads ? nullptr : &s_cb;
+  gcq::Combine* const ct = solo || grid_threads < gcq::kCombineEnqMinThreads ? nullptr : &s_ct;
   uint64_t eo = 0, ef = 0, dok = 0, de = 0;
   if (tid < threads) {
     const uint64_t seq0 = (uint64_t)launch * chunk + 1;
@@ -406,7 +411,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
       for (uint32_t i = 0; i < chunk; ++i) {
         if (prod) {
           const uint64_t v = ((uint64_t)tid << 32) | (uint32_t)(seq0 + i);
-          if (q.enq(tid, v)) ++eo; else ++ef;
+          if (q.enq(tid, v, nullptr, ct)) ++eo; else ++ef;
         } else {
           uint64_t o;
           if (q.deq(tid, &o, nullptr, cb)) ++dok; else ++de;
@@ -415,7 +420,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
     } else {
       for (uint32_t i = 0; i < chunk; ++i) {
         const uint64_t v = ((uint64_t)tid << 32) | (uint32_t)(seq0 + i);
-        if (q.enq(tid, v)) ++eo; else ++ef;
+        if (q.enq(tid, v, nullptr, ct)) ++eo; else ++ef;
         uint64_t o;
         if (q.deq(tid, &o, nullptr, cb)) ++dok; else ++de;
       }
@@ -613,15 +618,18 @@ template <class Q>
 __global__ void k_fifo_mixed(Q q, uint32_t threads, uint32_t producers, FifoDev f, uint32_t ops, uint64_t total,
                              uint32_t* fail) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  __shared__ gcq::Combine s_cb;
-  if (threadIdx.x == 0) gcq::combine_init(&s_cb);
+  __shared__ gcq::Combine s_cb, s_ct;
+  if (threadIdx.x == 0) {
+    gcq::combine_init(&s_cb);
+    gcq::combine_init(&s_ct);
+  }
   __syncthreads();
   if (tid >= threads) return;
   if (tid < producers) {
     for (uint32_t i = 0; i < ops; ++i) {
       const uint64_t tok = ((uint64_t)tid << 32) | (uint64_t)(i + 1u);
       uint32_t spins = 0;
-      while (!q.enq(tid, tok)) {
+      while (!q.enq(tid, tok, nullptr, &s_ct)) {
         if (++spins > kFifoSpinLimit) {
           gcq::faa_rlx(fail, 1u);
           return;
@@ -759,8 +767,11 @@ static_assert(sizeof(LogEntry) == sizeof(gcq_hist_entry), "log entry layout");
 template <class Q>
 __global__ void k_hist(Q q, gcq_hist_config c, uint32_t stride, LogEntry* log, uint64_t* clock) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  __shared__ gcq::Combine s_cb;
-  if (threadIdx.x == 0) gcq::combine_init(&s_cb);
+  __shared__ gcq::Combine s_cb, s_ct;
+  if (threadIdx.x == 0) {
+    gcq::combine_init(&s_cb);
+    gcq::combine_init(&s_ct);
+  }
   __syncthreads();
   if (tid >= c.threads) return;
   const bool victim = c.delay_every && ((tid + (uint32_t)c.seed) % c.delay_every == 0);
@@ -779,7 +790,7 @@ __global__ void k_hist(Q q, gcq_hist_config c, uint32_t stride, LogEntry* log, u
       e.op = 0;
       e.arg = v;
       e.call = gcq::faa_rlx(clock, (uint64_t)1);
-      const bool ok = q.enq(tid, v, &p);
+      const bool ok = q.enq(tid, v, &p, &s_ct);
       e.ret_time = gcq::faa_rlx(clock, (uint64_t)1);
       e.ret = ok ? 1 : 0;
       e.steps = p.steps;

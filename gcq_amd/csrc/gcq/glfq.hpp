@@ -180,13 +180,15 @@ struct NoProbe {
 
 // Enqueue a slot index; the ring must have room (indirection guarantees it).
 // scq_reset: reset the threshold in SCQ's load-then-store form (the
-// free-index ring; reset_threshold).
+// free-index ring; reset_threshold).  ct: draw the Tail ticket together with
+// the workgroup's other enqueuing waves (group_faa).
 template <class Probe = NoProbe>
-__device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool scq_reset = false) {
+__device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool scq_reset = false,
+                               Combine* ct = nullptr) {
   const Geometry g = r->g;
   uint64_t* const e = slots(r);
   for (;;) {
-    const uint64_t T = wave_faa(tail(r), (uint64_t)1);
+    const uint64_t T = ct ? group_faa(ct, tail(r), (uint64_t)1) : wave_faa(tail(r), (uint64_t)1);
     // Every earlier memory operation of this wave completes before the slot
     // is touched; this is what orders a payload store before the index is
     // published (and a payload load before the index is recycled).
@@ -321,11 +323,11 @@ __host__ __device__ inline Mpmc mpmc_view(void* base, uint32_t n) {
 }
 
 template <class Probe = NoProbe>
-__device__ inline bool enqueue_ptr(const Mpmc& q, uint64_t v, Probe* pr = nullptr) {
+__device__ inline bool enqueue_ptr(const Mpmc& q, uint64_t v, Probe* pr = nullptr, Combine* ct = nullptr) {
   const uint32_t idx = dequeue(q.fq, pr, false);
   if (idx == kBot) return false;  // full
   st_rlx(&q.data[idx], v);  // acknowledged before enqueue() touches a slot
-  enqueue(q.aq, idx, pr);
+  enqueue(q.aq, idx, pr, false, ct);
   return true;
 }
 

@@ -257,9 +257,11 @@ struct RingOps {
 
   // Fast path enqueue attempt.  0 = done, else the raw ticket that failed.
   // scq_reset: reset the threshold in SCQ's load-then-store form (the
-  // free-index ring; reset_threshold).
-  __device__ __forceinline__ uint64_t try_enq(uint32_t idx, bool scq_reset = false) {
-    const uint64_t T = cnt_of(wave_faa(tail(r), (uint64_t)1 << g.thr_bits));
+  // free-index ring; reset_threshold).  ct: draw the Tail ticket together
+  // with the workgroup's other enqueuing waves (group_faa).
+  __device__ __forceinline__ uint64_t try_enq(uint32_t idx, bool scq_reset = false, Combine* ct = nullptr) {
+    const uint64_t step = (uint64_t)1 << g.thr_bits;
+    const uint64_t T = cnt_of(ct ? group_faa(ct, tail(r), step) : wave_faa(tail(r), step));
     // Every earlier memory operation of this wave completes before the slot
     // is touched: this orders a payload store before its index is published
     // and a payload load before its index is recycled.
@@ -602,11 +604,11 @@ struct RingOps {
   }
 
   // Enqueue_wf (gwf_ring.hpp:1234-1284).
-  __device__ void enqueue(uint32_t idx, bool scq_reset = false) {
+  __device__ void enqueue(uint32_t idx, bool scq_reset = false, Combine* ct = nullptr) {
     help_threads();
     uint64_t T = 0;
     for (uint32_t c = g.enq_patience; --c != 0;) {
-      T = try_enq(idx, scq_reset);
+      T = try_enq(idx, scq_reset, ct);
       if (T == 0) return;
     }
     pr->slow_entry();
@@ -723,13 +725,14 @@ __host__ __device__ inline Mpmc mpmc_view(void* base, uint32_t n, uint32_t max_t
 }
 
 template <class Probe = NoProbe>
-__device__ inline bool enqueue_ptr(const Mpmc& q, uint32_t tid, uint64_t v, Probe* p = nullptr) {
+__device__ inline bool enqueue_ptr(const Mpmc& q, uint32_t tid, uint64_t v, Probe* p = nullptr,
+                                   Combine* ct = nullptr) {
   NoProbe np;
   Probe* pp = p ? p : (Probe*)&np;
   const uint32_t idx = RingOps<Probe>(q.fq, tid, pp).dequeue(false);
   if (idx == kBot) return false;  // full
   st_rlx(&q.data[idx], v);  // acknowledged before try_enq touches a slot
-  RingOps<Probe>(q.aq, tid, pp).enqueue(idx);
+  RingOps<Probe>(q.aq, tid, pp).enqueue(idx, false, ct);
   return true;
 }
 

@@ -23,14 +23,18 @@ struct gcq_queue {
 namespace gcqk {
 
 // deq's Combine argument: the kernel's LDS context for combining the
-// consumer-side claims of a workgroup's idle waves (G-WFQ, G-LFQ; the other
-// queues take no bounded claim and ignore it).
+// consumer-side claims of a workgroup's idle waves.  enq's: a second LDS
+// context for drawing the producers' Tail tickets together (group_faa).
+// G-WFQ and G-LFQ use both; the other queues ignore them.
 
 struct GwfQ {
   gcq::gwf::Mpmc m;
   template <class P = gcq::gwf::NoProbe>
-  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr) const {
-    return gcq::gwf::enqueue_ptr<P>(m, tid, v, p);
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
+    return gcq::gwf::enqueue_ptr<P>(m, tid, v, p, ct);
+  }
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, decltype(nullptr), gcq::Combine* ct) const {
+    return enq(tid, v, (gcq::gwf::NoProbe*)nullptr, ct);
   }
   template <class P = gcq::gwf::NoProbe>
   __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr, gcq::Combine* cb = nullptr) const {
@@ -44,8 +48,11 @@ struct GwfQ {
 struct LfqQ {
   gcq::lfq::Mpmc m;
   template <class P = gcq::lfq::NoProbe>
-  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr) const {
-    return gcq::lfq::enqueue_ptr<P>(m, v, p);
+  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
+    return gcq::lfq::enqueue_ptr<P>(m, v, p, ct);
+  }
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, decltype(nullptr), gcq::Combine* ct) const {
+    return enq(tid, v, (gcq::lfq::NoProbe*)nullptr, ct);
   }
   template <class P = gcq::lfq::NoProbe>
   __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr, gcq::Combine* cb = nullptr) const {
@@ -65,8 +72,11 @@ struct SfqQ {
   gcq::sfq::Queue* q;
   uint32_t exact_empty;
   template <class P = gcq::sfq::NoProbe>
-  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr) const {
+  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr, gcq::Combine* = nullptr) const {
     return gcq::sfq::enqueue_nb(q, v, p) == gcq::sfq::kSuccess;
+  }
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, decltype(nullptr), gcq::Combine* ct) const {
+    return enq(tid, v, (gcq::sfq::NoProbe*)nullptr, ct);
   }
   template <class P = gcq::sfq::NoProbe>
   __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr, gcq::Combine* = nullptr) const {
@@ -82,8 +92,11 @@ struct YmcQ {
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
   template <class P = gcq::ymc::NoProbe>
-  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr) const {
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr, gcq::Combine* = nullptr) const {
     return gcq::ymc::enqueue(q, h, tid, v, p);
+  }
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, decltype(nullptr), gcq::Combine* ct) const {
+    return enq(tid, v, (gcq::ymc::NoProbe*)nullptr, ct);
   }
   template <class P = gcq::ymc::NoProbe>
   __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, P* p = nullptr, gcq::Combine* = nullptr) const {

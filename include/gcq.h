@@ -116,9 +116,10 @@ typedef struct gcq_tb_config {
   uint32_t launches_per_sync; /* async launches between host time checks (0 = auto) */
   uint32_t warmup_launches;   /* > 0 with timed_launches: launch-count mode instead */
   uint32_t timed_launches;    /* exactly this many timed launches (0 = time mode) */
-  uint32_t solo_claims;       /* 1 = no workgroup combining of idle consumers'
-                                 claims (G-WFQ, G-LFQ); 0 (default) = combine
-                                 in grids of 8,192 threads or more */
+  uint32_t solo_claims;       /* 1 = no workgroup combining (G-WFQ, G-LFQ);
+                                 0 (default) = idle consumers' claims combined
+                                 in grids of 8,192 threads or more, producers'
+                                 Tail tickets in grids of 32,768 or more */
 } gcq_tb_config;
 typedef struct gcq_tb_result {
   uint64_t enq_success, enq_fail, deq_success, deq_empty;

@@ -14,11 +14,15 @@
 // Built with -DPLANT_LATE_CLOSE the leader reads Tail before it closes its
 // group, the mistake the ordering rule exists to prevent; the member check
 // must then fail.
+// With a third argument "faa" the leader instead adds the group's whole lane
+// count, unbounded (group_faa, the producers' Tail tickets): every lane then
+// gets a ticket, and the tickets must still form exactly [0, Head).
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -35,6 +39,7 @@ struct Combine {
 std::atomic<uint64_t> g_head{0}, g_tail{0}, g_event{0};
 std::atomic<uint64_t> g_call_event[16];
 std::atomic<uint64_t> g_failures{0};
+bool g_plain_faa = false;
 constexpr uint64_t kSpinLimit = 1ull << 28;
 
 void fail(const char* what) {
@@ -72,7 +77,7 @@ bool claim(Combine* c, uint32_t w, uint32_t n, uint64_t* out, std::mt19937& rng)
     for (uint32_t b = members; b; b &= b - 1)
       if (g_call_event[__builtin_ctz(b)].load() >= tread) fail("a member started after the leader read Tail");
     const uint64_t avail = t > h ? t - h : 0;
-    k = avail < lanes ? (uint32_t)avail : lanes;
+    k = g_plain_faa ? lanes : (avail < lanes ? (uint32_t)avail : lanes);
     if (k) base = g_head.fetch_add(k);
     for (uint32_t b = members & ~(1u << w); b; b &= b - 1) {
       const uint32_t x = (uint32_t)__builtin_ctz(b);
@@ -98,6 +103,7 @@ bool claim(Combine* c, uint32_t w, uint32_t n, uint64_t* out, std::mt19937& rng)
 int main(int argc, char** argv) {
   const int trials = argc > 1 ? std::atoi(argv[1]) : 30;
   const int ops = argc > 2 ? std::atoi(argv[2]) : 2000;
+  g_plain_faa = argc > 3 && std::string(argv[3]) == "faa";
   uint64_t empties = 0, groups_with_items = 0;
   for (int trial = 0; trial < trials; ++trial) {
     g_head = 0;
@@ -125,8 +131,10 @@ int main(int argc, char** argv) {
         for (int i = 0; i < ops; ++i) {
           const uint32_t n = 1 + rng() % 64;
           if (claim(&c, (uint32_t)w, n, out, rng)) trial_empties++;
-          for (uint32_t j = 0; j < n; ++j)
+          for (uint32_t j = 0; j < n; ++j) {
             if (out[j] != ~0ull) got[w].push_back(out[j]);
+            else if (g_plain_faa) fail("a lane got no ticket from a plain group FAA");
+          }
         }
       });
     for (auto& t : th) t.join();

@@ -1,6 +1,7 @@
 """CPU model of the workgroup claim-combining protocol (combine_claim in
 gcq_amd/csrc/gcq/prim.hpp): tickets unique and dense, every group member's
-call before the leader's Tail read, no deadlock, across 2-16 waves.  The same
+call before the leader's Tail read, no deadlock, across 2-16 waves; and the
+same for the unbounded group FAA of producers' Tail tickets (group_faa).  The same
 model with the Tail read moved before the group's close (a planted ordering
 mistake) must be caught.  CPU only."""
 import os
@@ -33,3 +34,10 @@ def test_combine_model_catches_late_close(tmp_path):
     r = subprocess.run([exe, "30", "2000"], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "a member started after the leader read Tail" in r.stderr
+
+
+def test_group_faa_model(tmp_path):
+    exe = _build(tmp_path, "combine_model_faa")
+    r = subprocess.run([exe, "45", "3000", "faa"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "failures 0" in r.stdout
