@@ -390,19 +390,22 @@ namespace {
 // per wave (shuffles) and per block (LDS) into this block's own stats slot;
 // the reference instead issues four device-global atomicAdds per thread
 // (throughout_bench.cpp:836-839, 878-881).
-template <class Q>
+// kGroupEnq: producers draw Tail tickets per workgroup (group_faa).  A
+// template argument rather than a runtime pointer test, so that grids below
+// kCombineEnqMinThreads run the per-wave enqueue code unchanged (with the
+// test left in, G-LFQ split 25/75 at 1,024 threads measured 13% slower).
+template <class Q, bool kGroupEnq>
 __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32_t pp, uint32_t launch,
                      uint32_t solo, uint64_t* stats) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ gcq::Combine s_cb, s_ct;
   if (threadIdx.x == 0) {
     gcq::combine_init(&s_cb);
-    gcq::combine_init(&s_ct);
+    if (kGroupEnq) gcq::combine_init(&s_ct);
   }
   __syncthreads();
-  const uint32_t grid_threads = gridDim.x * blockDim.x;
-  gcq::Combine* const cb = solo || grid_threads < gcq::kCombineMinThreads ? nullptr : &s_cb;
-  gcq::Combine* const ct = solo || grid_threads < gcq::kCombineEnqMinThreads ? nullptr : &s_ct;
+  gcq::Combine* const cb = solo || gridDim.x * blockDim.x < gcq::kCombineMinThreads ? nullptr : &s_cb;
+  gcq::Combine* const ct = kGroupEnq ? &s_ct : nullptr;
   uint64_t eo = 0, ef = 0, dok = 0, de = 0;
   if (tid < threads) {
     const uint64_t seq0 = (uint64_t)launch * chunk + 1;
@@ -466,9 +469,14 @@ extern "C" int gcq_tb_run(gcq_queue* q, const gcq_tb_config* cfg, gcq_tb_result*
       q,
       [&](auto a) -> int {
         using A = decltype(a);
+        const bool group_enq = !cfg->solo_claims && grid * cfg->block >= gcq::kCombineEnqMinThreads;
         auto go = [&]() {
-          hipLaunchKernelGGL(k_tb<A>, dim3(grid), dim3(cfg->block), 0, q->stream, a, cfg->threads, cfg->chunk_ops,
-                             cfg->mode, cfg->producer_percent, launch++, cfg->solo_claims, stats);
+          if (group_enq)
+            hipLaunchKernelGGL((k_tb<A, true>), dim3(grid), dim3(cfg->block), 0, q->stream, a, cfg->threads,
+                               cfg->chunk_ops, cfg->mode, cfg->producer_percent, launch++, cfg->solo_claims, stats);
+          else
+            hipLaunchKernelGGL((k_tb<A, false>), dim3(grid), dim3(cfg->block), 0, q->stream, a, cfg->threads,
+                               cfg->chunk_ops, cfg->mode, cfg->producer_percent, launch++, cfg->solo_claims, stats);
         };
         using clk = std::chrono::steady_clock;
         // Warmup: host-timed like the reference (throughout_bench.cpp:1010-1021).
