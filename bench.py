@@ -156,6 +156,7 @@ def main():
                                "25% producers, 128 ops/thread/launch, capacity 2^20",
                    "queue": args.queue, "threads": T, "block": 256, "chunk_ops": 128, "producer_percent": 25,
                    "capacity": 1 << 20, "poll_backoff": 0, "claims": "combined per workgroup when idle",
+                   "producer_tickets": "drawn per workgroup",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "atomic", "achieved": round(achieved_ops / 1e9, 4),
                      "peak": round(p1["tickets_per_s"] / 1e9, 4), "unit": "Gop/s",
