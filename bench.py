@@ -89,7 +89,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the queue path is single-GPU; see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "(each queue lives in one GPU's memory; several GPUs would each run their own queues)",
+              file=sys.stderr)
         sys.exit(2)
 
     import gcq_amd as g
