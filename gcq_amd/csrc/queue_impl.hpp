@@ -25,7 +25,7 @@ namespace gcqk {
 // deq's Combine argument: the kernel's LDS context for combining the
 // consumer-side claims of a workgroup's idle waves.  enq's: a second LDS
 // context for drawing the producers' Tail tickets together (group_faa).
-// G-WFQ and G-LFQ use both; the other queues ignore them.
+// G-WFQ and G-LFQ use both, WFQ only the second; SFQ ignores them.
 
 struct GwfQ {
   gcq::gwf::Mpmc m;
@@ -92,8 +92,8 @@ struct YmcQ {
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
   template <class P = gcq::ymc::NoProbe>
-  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr, gcq::Combine* = nullptr) const {
-    return gcq::ymc::enqueue(q, h, tid, v, p);
+  __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
+    return gcq::ymc::enqueue(q, h, tid, v, p, ct);
   }
   __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, decltype(nullptr), gcq::Combine* ct) const {
     return enq(tid, v, (gcq::ymc::NoProbe*)nullptr, ct);
