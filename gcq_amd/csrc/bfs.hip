@@ -70,14 +70,28 @@ struct LevelAcc {
 // 18% slower without it.
 constexpr int32_t kProbeEdges = 256;
 
+// One pop round per level: every wave of the grid starts a level with one
+// pop of all 64 lanes, so that first round takes the tickets [h0, h0 +
+// lanes).  When the level's tickets [h0, end) fit in it, no later pop can
+// find an item, and a wave leaves after its first batch instead of paying
+// one more FAA round trip to learn that.  h0 = kNoHead when the level's
+// starting Head is not known.
+constexpr uint64_t kNoHead = ~0ull;
+
 template <bool kCounted, class F>
 __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restrict__ offs,
                              const int32_t* __restrict__ idx, int32_t* level, int32_t L, uint32_t tid, uint32_t* stage,
-                             LevelAcc& acc, uint32_t* err) {
+                             LevelAcc& acc, uint32_t* err, uint64_t h0 = kNoHead, uint64_t grid_lanes = 0,
+                             uint64_t* end_out = nullptr) {
   const uint32_t lane = lane_id();
   uint32_t sc = 0;  // staged vertices (wave-uniform)
   uint64_t end = 0;
-  if constexpr (kCounted) end = qin.level_end();
+  bool one_round = false;
+  if constexpr (kCounted) {
+    end = qin.level_end();
+    if (end_out) *end_out = end;
+    one_round = h0 != kNoHead && end >= h0 && end - h0 <= grid_lanes;
+  }
   for (;;) {
     uint32_t u = 0;
     bool got;
@@ -132,6 +146,7 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
       acc.disc += disc ? 1u : 0u;
       if (sc >= 64) flush(qout, tid, stage, sc, 64, err);
     }
+    if (one_round) break;
   }
   if (sc) flush(qout, tid, stage, sc, sc, err);
 }
@@ -155,11 +170,19 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
   uint32_t* st = stage[(threadIdx.x / 64) & 15];
   LevelAcc acc{0, 0};
   uint64_t disc_total = 0;
+  uint64_t prev_end[2] = {kNoHead, kNoHead};  // per frontier: its end when last drained
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
     acc.disc = 0;
-    expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err);
+    if constexpr (F::kHeadFromDrain) {
+      uint64_t end = 0;
+      expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
+                                prev_end[L & 1], nthreads, &end);
+      prev_end[L & 1] = end;
+    } else {
+      expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err);
+    }
     disc_total += acc.disc;
     uint32_t n = 0;
     if (!level_barrier(&c->lbar, gridDim.x, L, wave_sum32(acc.disc), &n, [&]() {

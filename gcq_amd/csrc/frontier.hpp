@@ -26,9 +26,13 @@ constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
 // on Head compared with that end: no Head/Tail claim reads, and the level's
 // last pop costs one FAA.  Head overshoots the end by up to a wave's lanes per
 // wave; drained() at the barrier sets it back to the end.
+// kHeadFromDrain: drained() leaves Head equal to the level's end, so a ring's
+// Head at the start of the next level it feeds is that end, known to every
+// wave (two frontiers alternate, so it is the end of two levels before).
 struct FGwf {
   static constexpr int kQueues = 2;
   static constexpr bool kCounted = true;
+  static constexpr bool kHeadFromDrain = true;
   gcq::gwf::Ring* r;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
@@ -76,6 +80,7 @@ struct FGwf {
 struct FLfq {
   static constexpr int kQueues = 2;
   static constexpr bool kCounted = true;
+  static constexpr bool kHeadFromDrain = true;
   gcq::lfq::Ring* r;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
@@ -120,6 +125,7 @@ struct FLfq {
 struct FSfq {
   static constexpr int kQueues = 2;
   static constexpr bool kCounted = true;
+  static constexpr bool kHeadFromDrain = true;
   gcq::sfq::Queue* q;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
@@ -165,6 +171,7 @@ struct FSfq {
 struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt.hpp:42-46)
   static constexpr int kQueues = 3;
   static constexpr bool kCounted = true;
+  static constexpr bool kHeadFromDrain = false;  // rewound a level earlier instead
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
   // Counted pops: a level's input queue was only enqueued into during the
