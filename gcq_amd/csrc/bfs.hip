@@ -175,18 +175,14 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
     acc.disc = 0;
-    if constexpr (F::kHeadFromDrain) {
-      uint64_t end = 0;
-      expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
-                                prev_end[L & 1], nthreads, &end);
-      prev_end[L & 1] = end;
-    } else {
-      expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err);
-    }
+    uint64_t end = 0;
+    expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
+                              F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads, &end);
+    prev_end[L & 1] = end;
     disc_total += acc.disc;
     uint32_t n = 0;
     if (!level_barrier(&c->lbar, gridDim.x, L, wave_sum32(acc.disc), &n, [&]() {
-          if constexpr (F::kCounted) fq.in(L).drained();
+          if constexpr (F::kCounted) fq.in(L).drained(end);
           if (F::kQueues == 3) fq.idle(L).rewind();
         }))
       break;

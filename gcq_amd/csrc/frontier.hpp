@@ -57,10 +57,11 @@ struct FGwf {
     *v = e_idx(E);
     return 1;
   }
-  __device__ __forceinline__ void drained() const {
+  // end: the level's end (level_end() at its start): the input ring's Tail,
+  // which no one moves during the level, so the barrier need not read it.
+  __device__ __forceinline__ void drained(uint64_t end) const {
     const uint32_t B = r->g.thr_bits;
-    const uint64_t t = gcq::ld_rlx(gcq::gwf::tail(r)) >> B;
-    gcq::st_rlx(gcq::gwf::head(r), (t << B) | ((1ull << B) - 1ull));
+    gcq::st_rlx(gcq::gwf::head(r), (end << B) | ((1ull << B) - 1ull));
   }
   __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
     gcq::gwf::NoProbe np;
@@ -108,9 +109,7 @@ struct FLfq {
     *v = e_idx(E);
     return 1;
   }
-  __device__ __forceinline__ void drained() const {
-    gcq::st_rlx(gcq::lfq::head(r), gcq::ld_rlx(gcq::lfq::tail(r)));
-  }
+  __device__ __forceinline__ void drained(uint64_t end) const { gcq::st_rlx(gcq::lfq::head(r), end); }
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     gcq::lfq::enqueue(r, v);
     return true;
@@ -151,9 +150,7 @@ struct FSfq {
     *v = (uint32_t)x;
     return 1;
   }
-  __device__ __forceinline__ void drained() const {
-    gcq::st_rlx(gcq::sfq::head(q), gcq::ld_rlx(gcq::sfq::tail(q)));
-  }
+  __device__ __forceinline__ void drained(uint64_t end) const { gcq::st_rlx(gcq::sfq::head(q), end); }
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     return gcq::sfq::enqueue_blocking(q, (uint64_t)v, 1ull << 22) == gcq::sfq::kSuccess;
   }
@@ -196,7 +193,7 @@ struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt
     *v = (uint32_t)(x - 1);
     return 1;
   }
-  __device__ __forceinline__ void drained() const {}
+  __device__ __forceinline__ void drained(uint64_t) const {}
   __device__ __forceinline__ void recycle(uint64_t gid, uint64_t stride, uint32_t tid) const {
     gcq::ymc::recycle_cells(q, gid, stride);
     gcq::ymc::reset_handle(q, h, tid);
