@@ -90,6 +90,21 @@ def test_throughput_conserves_items(gcq, kind, solo):
     assert r["deq_success"] <= r["enq_success"] + (1 << 16) + 8192 * 32
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2], ids=["gwfq", "glfq", "wfq"])
+@pytest.mark.parametrize("mode", [1, 0], ids=["split25", "balanced"])
+def test_grouped_tail_tickets_conserve_items(gcq, kind, mode):
+    # 32,768 threads: the smallest grid in which the throughput kernel draws
+    # producers' Tail tickets per workgroup (the headline kernel's form).
+    print(f"[tb grouped] kind={NAMES[kind]} mode={mode}", flush=True)
+    kw = {"wfq_segment_cells": 1 << 26} if kind == 2 else {}
+    with gcq.Queue(kind, capacity=1 << 20, max_threads=32768, poll_backoff=0, **kw) as q:
+        r = q.tb_run(mode, threads=32768, chunk_ops=32, producer_percent=25, warmup_launches=2, timed_launches=5)
+    print(f"[tb grouped] succ {r['succ_mops']:.1f} Mops/s, enq {r['enq_success']} deq {r['deq_success']}", flush=True)
+    assert r["enq_fail"] == 0
+    assert r["enq_success"] == (32768 // 4 if mode == 1 else 32768) * 32 * 5
+    assert 0 < r["deq_success"] <= r["enq_success"] + 32768 * 32 * 2
+
+
 @pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
 @pytest.mark.parametrize("pp", [25, 75])
 def test_combined_claims_split_histories(gcq, orc, kind, pp):
