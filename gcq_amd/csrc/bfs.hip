@@ -117,7 +117,7 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
     const int32_t excl = incl - deg;
     const int32_t total = __shfl(incl, 63, 64);
     const bool probe = total > kProbeEdges;
-    for (int32_t base = 0; base < total; base += 64) {
+    auto neighbour = [&](int32_t base) -> uint32_t {  // this lane's edge target in a round, 0 past the end
       const int32_t e = base + (int32_t)lane;
       const int32_t ec = e < total ? e : total - 1;
       int lo = 0, hi = 63;  // owner = first lane whose inclusive prefix exceeds ec
@@ -129,10 +129,19 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
       }
       const int32_t obeg = __shfl(beg, lo, 64);
       const int32_t oexcl = __shfl(excl, lo, 64);
+      return e < total ? (uint32_t)idx[obeg + (e - oexcl)] : 0u;
+    };
+    // The next round's neighbour indices are loaded before this round's
+    // CASes, so the two latencies overlap.
+    uint32_t w_next = total > 0 ? neighbour(0) : 0u;
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t e = base + (int32_t)lane;
+      const uint32_t w_here = w_next;
+      if (base + 64 < total) w_next = neighbour(base + 64);
       bool disc = false;
       uint32_t w = 0;
       if (e < total) {
-        w = (uint32_t)idx[obeg + (ec - oexcl)];
+        w = w_here;
         // A depth changes once per traversal, from -1 to a level, so a cached
         // (possibly stale) read can only send a visited vertex to the CAS.
         if (!probe || gcq::ld_cached(&level[w]) == -1) {
