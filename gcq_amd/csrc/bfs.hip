@@ -180,9 +180,16 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
   LevelAcc acc{0, 0};
   uint64_t disc_total = 0;
   uint64_t prev_end[2] = {kNoHead, kNoHead};  // per frontier: its end when last drained
+  uint32_t cum = 0;                            // discoveries over all levels so far
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
+    // The frontier drained at the last barrier is this level's output and
+    // is read again only at the next level, so its Head is set back to its
+    // end here, off the barrier's path; this block's next arrival orders the
+    // store before that level.
+    if constexpr (F::kCounted && F::kQueues == 2)
+      if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
     acc.disc = 0;
     uint64_t end = 0;
     expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
@@ -190,9 +197,13 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
     prev_end[L & 1] = end;
     disc_total += acc.disc;
     uint32_t n = 0;
-    if (!level_barrier(&c->lbar, gridDim.x, L, wave_sum32(acc.disc), &n, [&]() {
-          if constexpr (F::kCounted) fq.in(L).drained(end);
-          if (F::kQueues == 3) fq.idle(L).rewind();
+    if (!level_barrier(&c->lbar, gridDim.x, L, wave_sum32(acc.disc), &n, &cum, [&]() {
+          if constexpr (F::kQueues == 3) {
+            fq.idle(L).rewind();
+            return true;
+          } else {
+            return false;
+          }
         }))
       break;
     if (n == 0) break;

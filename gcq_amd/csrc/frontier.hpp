@@ -292,12 +292,17 @@ struct LevelBarrier {
 };
 
 // All threads of the block call it; wave_disc is the calling wave's
-// discovery count (wave-uniform).  *next gets the next level's size.  The last
-// block to arrive runs last() first.  Returns false once any block has
-// aborted (spin limit).
+// discovery count (wave-uniform).  *next gets the next level's size and
+// *cum (per thread, the same in every thread) the running total of the
+// levels' sizes.  The arrival word only grows: [63:32] counts arrivals over
+// all levels and [31:0] the discoveries over all levels (below 2^32: each
+// vertex is discovered once), so the last block needs no reset store.  The
+// last block to arrive runs last() first; when last() returns true it made
+// stores that must land before the release.  Returns false once any block
+// has aborted (spin limit).
 template <class Last>
 __device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint32_t wave_disc, uint32_t* next,
-                              Last last) {
+                              uint32_t* cum, Last last) {
   __shared__ uint32_t s_wd[16];
   __shared__ uint32_t s_next, s_ok;
   gcq::vm_wait();  // this wave's pushes have completed
@@ -308,11 +313,9 @@ __device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint
     for (uint32_t k = 0; k < (blockDim.x + 63) / 64; ++k) sum += s_wd[k];
     const uint64_t old = gcq::faa_rlx(&b->arrive, (1ull << 32) | sum);
     uint32_t ok = 1, n = 0;
-    if ((uint32_t)(old >> 32) == nblocks - 1) {
-      n = (uint32_t)old + sum;
-      last();
-      gcq::st_rlx(&b->arrive, 0ull);
-      gcq::vm_wait();  // reset (and last()) complete before the release
+    if ((uint32_t)(old >> 32) == (uint32_t)(L + 1) * nblocks - 1u) {
+      n = (uint32_t)old + sum - *cum;
+      if (last()) gcq::vm_wait();  // last()'s stores complete before the release
       gcq::st_rlx(&b->release, ((uint64_t)(uint32_t)(L + 1) << 32) | n);
     } else {
       for (uint64_t spins = 0;; ++spins) {
@@ -334,6 +337,7 @@ __device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint
   }
   __syncthreads();
   *next = s_next;
+  *cum += s_next;
   return s_ok != 0;
 }
 
