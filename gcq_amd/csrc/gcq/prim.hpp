@@ -278,8 +278,8 @@ __device__ __forceinline__ uint32_t combine_claim(Combine* c, uint32_t lanes, ui
 // idle consumers poll, so fewer FAAs there shorten every wave's wait on it.
 // Measured, G-WFQ, block 256, grouped over per-wave FAAs: split 25/75 at
 // 8,192 / 16,384 / 65,536 threads 0.96 / 0.99 / 1.11, balanced at 65,536
-// 1.04.  Combining the free-index ring's Head FAA as well lost 6% at
-// 65,536 (only producers use that line).
+// 1.04.  Grouping the free-index ring's Head FAA as well (only producers
+// use that line) ran 16% slower at 65,536 than grouping the Tail FAA alone.
 constexpr uint32_t kCombineEnqMinThreads = 32768;
 template <class T>
 __device__ __forceinline__ T group_faa(Combine* c, T* p, nd_t<T> step) {
