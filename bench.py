@@ -102,7 +102,7 @@ def main():
     p1 = g.faa_peak(threads=T, block=256, counters=1, stride_bytes=128, aggregate=True, iters=64, launches=10)
     p4 = g.faa_peak(threads=T, block=256, counters=4, stride_bytes=128, aggregate=True, iters=64, launches=10)
     peaks = {"striped_gtickets_per_s": {}, "per_lane_1_counter_gtickets_per_s": None}
-    for k in (2, 8, 16, 64):
+    for k in (2, 8, 16, 32, 64):
         pk = g.faa_peak(threads=T, block=256, counters=k, stride_bytes=128, aggregate=True, iters=64, launches=5)
         peaks["striped_gtickets_per_s"][str(k)] = round(pk["tickets_per_s"] / 1e9, 3)
     peaks["striped_gtickets_per_s"]["1"] = round(p1["tickets_per_s"] / 1e9, 3)
