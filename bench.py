@@ -125,8 +125,9 @@ def main():
     # Consumers poll without sleeping between empty attempts (poll_backoff 0):
     # measured fastest for this split (backoff 1 and 2 are 6% and 9% slower
     # without combining, equal within 1% with it).  Idle consumer waves of a
-    # workgroup combine their claims (the library's default); the same run
-    # with solo claims is reported beside it.
+    # workgroup combine their claims and producer waves draw their Tail
+    # tickets together (the library's default); the same run without either
+    # (solo_claims) is reported beside it.
     with g.Queue(kind, capacity=1 << 20, max_threads=max(T, 64), poll_backoff=0) as q:
         r = q.tb_run(g.SPLIT, threads=T, block=256, chunk_ops=128, producer_percent=25,
                      warmup_launches=args.warmup, timed_launches=args.steps)
