@@ -237,7 +237,15 @@ template <class F>
 __device__ __forceinline__ uint32_t combine_claim(Combine* c, uint32_t lanes, uint32_t* k, uint64_t* base,
                                                   F&& memclaim) {
   const uint32_t w = wave_in_block();
-  const uint64_t old = __hip_atomic_fetch_add(&c->join, (1ull << (32 + w)) + lanes, __ATOMIC_RELAXED,
+  // Group ids are 16 bits wide.  A wave that was last a member of group X
+  // and next joins group X + 65,536 (having led or sat out every group in
+  // between) would otherwise find res_g[w] == X already and take that old
+  // group's result.  Only the leader of a group this wave joined writes
+  // res_g[w], and that leader has already written it for the wave's last
+  // membership, so the clear cannot race; the release join orders it before
+  // the leader's close.
+  __hip_atomic_store(&c->res_g[w], ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const uint64_t old = __hip_atomic_fetch_add(&c->join, (1ull << (32 + w)) + lanes, __ATOMIC_RELEASE,
                                               __HIP_MEMORY_SCOPE_WORKGROUP);
   const uint32_t grp = (uint32_t)(old >> 48);
   const uint32_t off = (uint32_t)old;

@@ -17,6 +17,12 @@
 // With a third argument "faa" the leader instead adds the group's whole lane
 // count, unbounded (group_faa, the producers' Tail tickets): every lane then
 // gets a ticket, and the tickets must still form exactly [0, Head).
+// With a third argument "wrap" a scripted run wraps the 16-bit group id: wave
+// 0 joins a group that wave 1 leads, then leads 65,535 groups alone, then
+// joins wave 1's next group, whose id equals the first one's modulo 2^16.
+// Built with -DPLANT_STALE_GROUP a wave does not clear its result word before
+// it joins; it then takes the first group's stale result and the ticket
+// checks must fail.
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
@@ -39,6 +45,7 @@ struct Combine {
 std::atomic<uint64_t> g_head{0}, g_tail{0}, g_event{0};
 std::atomic<uint64_t> g_call_event[16];
 std::atomic<uint64_t> g_failures{0};
+std::atomic<uint32_t> g_hold_for{0};  // scripted runs: waves a leader waits for before its close
 bool g_plain_faa = false;
 constexpr uint64_t kSpinLimit = 1ull << 28;
 
@@ -50,7 +57,10 @@ void fail(const char* what) {
 // Returns true when the group answered EMPTY.
 bool claim(Combine* c, uint32_t w, uint32_t n, uint64_t* out, std::mt19937& rng) {
   g_call_event[w].store(g_event.fetch_add(1));
-  const uint64_t old = c->join.fetch_add((1ull << (32 + w)) + n);
+#ifndef PLANT_STALE_GROUP
+  c->res_g[w].store(~0u, std::memory_order_relaxed);
+#endif
+  const uint64_t old = c->join.fetch_add((1ull << (32 + w)) + n, std::memory_order_release);
   const uint32_t grp = (uint32_t)(old >> 48);
   const uint32_t off = (uint32_t)old;
   uint32_t k = 0;
@@ -60,6 +70,8 @@ bool claim(Combine* c, uint32_t w, uint32_t n, uint64_t* out, std::mt19937& rng)
       if (s > kSpinLimit) { fail("leader waited forever"); break; }
       std::this_thread::yield();
     }
+    for (uint32_t hold = g_hold_for.load(); hold && ((c->join.load() >> 32) & hold) != hold;)
+      std::this_thread::yield();
     const uint64_t h = g_head.load();
 #ifdef PLANT_LATE_CLOSE
     const uint64_t tread = g_event.fetch_add(1);
@@ -98,9 +110,55 @@ bool claim(Combine* c, uint32_t w, uint32_t n, uint64_t* out, std::mt19937& rng)
   return k == 0;
 }
 
+// Scripted 16-bit group-id wrap (see the header).  Returns the failure count.
+uint64_t wrap_run() {
+  g_head = 0;
+  g_tail = 1ull << 50;  // every claim finds items
+  Combine c;
+  c.join = 1ull << 48;
+  c.done = 0;
+  for (auto& x : c.res_g) x = ~0u;
+  std::mt19937 rng(7);
+  std::vector<uint64_t> got;
+  uint64_t out[64];
+  auto keep = [&](uint32_t n) {
+    for (uint32_t j = 0; j < n; ++j)
+      if (out[j] != ~0ull) got.push_back(out[j]);
+  };
+  auto pair = [&]() {  // wave 1 leads, wave 0 joins as a member
+    g_hold_for = 1u;
+    uint64_t out1[64];
+    std::mt19937 r1(11);
+    std::thread lead([&] { claim(&c, 1, 5, out1, r1); });
+    while (!((c.join.load() >> 33) & 1u)) std::this_thread::yield();
+    claim(&c, 0, 3, out, rng);
+    lead.join();
+    g_hold_for = 0;
+    keep(3);
+    for (uint32_t j = 0; j < 5; ++j)
+      if (out1[j] != ~0ull) got.push_back(out1[j]);
+  };
+  pair();
+  for (int i = 0; i < 65535; ++i) {
+    claim(&c, 0, 2, out, rng);
+    keep(2);
+  }
+  pair();
+  std::vector<uint8_t> seen(g_head.load(), 0);
+  for (uint64_t t : got) {
+    if (t >= seen.size()) fail("ticket at or beyond Head");
+    else if (seen[t]++) fail("ticket handed out twice");
+  }
+  if (got.size() != g_head.load()) fail("tickets below Head not all handed out");
+  std::printf("wrap run: %zu tickets, Head %llu, failures %llu\n", got.size(), (unsigned long long)g_head.load(),
+              (unsigned long long)g_failures.load());
+  return g_failures.load();
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+  if (argc > 3 && std::string(argv[3]) == "wrap") return wrap_run() ? 1 : 0;
   const int trials = argc > 1 ? std::atoi(argv[1]) : 30;
   const int ops = argc > 2 ? std::atoi(argv[2]) : 2000;
   g_plain_faa = argc > 3 && std::string(argv[3]) == "faa";
