@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <filesystem>
 #include <mutex>
 #include <vector>
 
@@ -882,6 +883,14 @@ extern "C" int gcq_hist_run(gcq_queue* q, const gcq_hist_config* cfg, gcq_hist_e
 extern "C" int gcq_hist_write_jsonl(const gcq_hist_entry* e, size_t n, const char* path) {
   GCQ_REQUIRE(e || n == 0, "null entries");
   GCQ_REQUIRE(path, "null path");
+  // The parent directory is created as the reference's dump does
+  // (history.cpp:249-255).
+  const std::filesystem::path out_path(path);
+  if (out_path.has_parent_path()) {
+    std::error_code ec;
+    std::filesystem::create_directories(out_path.parent_path(), ec);
+    GCQ_REQUIRE(!ec, "cannot create the directory of %s", path);
+  }
   FILE* f = fopen(path, "wb");
   GCQ_REQUIRE(f, "cannot open %s", path);
   for (size_t i = 0; i < n; ++i)

@@ -550,3 +550,59 @@ extern "C" int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* i
   }
   return GCQ_OK;
 }
+
+// ======================================================================== image and CSV outputs
+// 8-bit binary PPM with the reference's quantisation: each channel becomes
+// (unsigned char)min(255, max(0, x * 255)), truncating (save_ppm,
+// rt_final.cpp:689-700; the compaction baseline writes the same format).
+extern "C" int gcq_rt_write_ppm(const float* img, int32_t w, int32_t h, const char* path) {
+  GCQ_REQUIRE(img && path && w > 0 && h > 0, "bad image or path");
+  FILE* f = fopen(path, "wb");
+  GCQ_REQUIRE(f, "cannot open %s", path);
+  fprintf(f, "P6\n%d %d\n255\n", w, h);
+  std::vector<unsigned char> row(3ull * w);
+  for (int32_t y = 0; y < h; ++y) {
+    const float* src = img + 3ll * w * y;
+    for (int64_t i = 0; i < 3ll * w; ++i) row[i] = (unsigned char)fminf(255.f, fmaxf(0.f, src[i] * 255.f));
+    if (fwrite(row.data(), 1, row.size(), f) != row.size()) {
+      fclose(f);
+      GCQ_REQUIRE(false, "short write to %s", path);
+    }
+  }
+  fclose(f);
+  return GCQ_OK;
+}
+
+// One result row appended to a CSV file, with its header when the file is new.
+//  GCQ_RT_CSV_QUEUE: the queue renderer's rt_queue_perf.csv header
+//    (rt_final.cpp:978-980), every column filled.  The reference's row omits
+//    PRIMARY, GEN_MS and TRACE_MS (:992-999); here GEN_MS is 0 because ray
+//    generation runs inside the one trace launch, and MRAYS is rays over
+//    trace time (:938).
+//  GCQ_RT_CSV_COMPACT: the compaction baseline's rt_performance.csv
+//    (baselines/rt_compaction.cpp:849-864), THREADS = block size, MRAYS over
+//    the frame time.
+extern "C" int gcq_rt_append_csv(const char* path, int32_t style, const gcq_rt_csv_row* r) {
+  GCQ_REQUIRE(path && r && r->res && r->gpu && r->queue, "null argument");
+  GCQ_REQUIRE(style == GCQ_RT_CSV_QUEUE || style == GCQ_RT_CSV_COMPACT, "unknown CSV style %d", style);
+  FILE* probe = fopen(path, "rb");
+  const bool exists = probe != nullptr;
+  if (probe) fclose(probe);
+  FILE* f = fopen(path, "ab");
+  GCQ_REQUIRE(f, "cannot open %s", path);
+  const gcq_rt_result& x = *r->res;
+  if (style == GCQ_RT_CSV_QUEUE) {
+    if (!exists) fprintf(f, "GPU,QUEUE,SCENE,THREADS,BOUNCES,PRIMARY,TOTAL,GEN_MS,TRACE_MS,TOTAL_MS,MRAYS\n");
+    const double mrays = x.kernel_ms > 0 ? (double)x.total_rays / 1e6 / (x.kernel_ms / 1e3) : 0.0;
+    fprintf(f, "\"%s\",%s,%s,%u,%d,%llu,%llu,%g,%g,%g,%g\n", r->gpu, r->queue, r->scene ? r->scene : "", r->threads,
+            r->bounces, (unsigned long long)x.primary_rays, (unsigned long long)x.total_rays, 0.0, x.kernel_ms,
+            x.total_ms, mrays);
+  } else {
+    if (!exists) fprintf(f, "DEVICE,QUEUE,THREADS,TOTAL_RAYS,TOTAL_TIME_MS,TOTAL_MRAYS\n");
+    const double mrays = x.kernel_ms > 0 ? (double)x.total_rays / 1e6 / (x.kernel_ms / 1e3) : 0.0;
+    fprintf(f, "%s,%s,%u,%llu,%g,%g\n", r->gpu, r->queue, r->threads, (unsigned long long)x.total_rays, x.kernel_ms,
+            mrays);
+  }
+  fclose(f);
+  return GCQ_OK;
+}

@@ -5,9 +5,31 @@ import ctypes as C
 import numpy as np
 
 from . import GLFQ, _check, load  # noqa: F401
-from .workloads import RtConfig, RtResult, RtSceneDesc
+from .workloads import RtConfig, RtCsvRow, RtResult, RtSceneDesc
 
 COMPLEX, CORNELL, RANDOM = 0, 1, 2
+CSV_QUEUE, CSV_COMPACT = 0, 1
+QUEUE_CSV_NAMES = {0: "GWFQ", 1: "GLFQ", 2: "WFQ", 3: "SFQ"}
+
+
+def write_ppm(img, path):
+    """8-bit binary PPM of an [h, w, 3] float image, truncated as the
+    reference's save_ppm does (rt_final.cpp:689-700)."""
+    a = np.ascontiguousarray(img, np.float32)
+    h, w, _ = a.shape
+    _check(load().gcq_rt_write_ppm(a.ctypes.data, w, h, str(path).encode()), "gcq_rt_write_ppm")
+
+
+def append_csv(path, style, result, gpu, queue, scene="", threads=0, bounces=0):
+    """Append one row (result: the dict render()/render_compact() return) in
+    the reference's rt_queue_perf.csv (CSV_QUEUE) or rt_performance.csv
+    (CSV_COMPACT) columns."""
+    res = RtResult()
+    for f, _ in RtResult._fields_:
+        if f in result:
+            setattr(res, f, result[f])
+    row = RtCsvRow(gpu.encode(), queue.encode(), scene.encode(), threads, bounces, C.pointer(res))
+    _check(load().gcq_rt_append_csv(str(path).encode(), style, C.byref(row)), "gcq_rt_append_csv")
 
 
 def config(**kw):

@@ -49,6 +49,11 @@ class RtResult(C.Structure):
                 ("total_ms", C.c_double), ("tiles", C.c_int32), ("pad", C.c_int32)]
 
 
+class RtCsvRow(C.Structure):
+    _fields_ = [("gpu", C.c_char_p), ("queue", C.c_char_p), ("scene", C.c_char_p), ("threads", C.c_uint32),
+                ("bounces", C.c_int32), ("res", C.POINTER(RtResult))]
+
+
 class Request(C.Structure):
     _fields_ = [("type", C.c_int32), ("graph", C.c_int32), ("src", C.c_int32), ("x0", C.c_int32), ("y0", C.c_int32),
                 ("w", C.c_int32), ("h", C.c_int32), ("keep", C.c_int32), ("id", C.c_uint64), ("arrival_ms", C.c_double)]
@@ -94,6 +99,8 @@ SIGNATURES = {
     "gcq_rt_config_default": (None, [_P(RtConfig)]),
     "gcq_rt_render": (C.c_int, [VP, _P(RtConfig), VP, _P(RtResult)]),
     "gcq_rt_render_compact": (C.c_int, [VP, _P(RtConfig), VP, _P(RtResult)]),
+    "gcq_rt_write_ppm": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_char_p]),
+    "gcq_rt_append_csv": (C.c_int, [C.c_char_p, C.c_int32, _P(RtCsvRow)]),
     "gcq_server_config_default": (None, [_P(ServerConfig)]),
     "gcq_server_create": (C.c_int, [_P(VP), C.c_int32, VP, C.c_int32, C.c_int32, _P(ServerConfig), _P(VP)]),
     "gcq_server_start": (C.c_int, [VP]),

@@ -337,6 +337,26 @@ int gcq_rt_render(gcq_rt_scene* s, const gcq_rt_config* cfg, float* img_out, gcq
  * ignored.  kernel_ms is the frame: first launch to last kernel end,
  * including the host's read of each bounce's ray count. */
 int gcq_rt_render_compact(gcq_rt_scene* s, const gcq_rt_config* cfg, float* img_out, gcq_rt_result* res);
+/* Write a w x h RGB float image as an 8-bit binary PPM, each channel
+ * truncated from min(255, max(0, x * 255)) as save_ppm does
+ * (rt_final.cpp:689-700). */
+int gcq_rt_write_ppm(const float* img, int32_t w, int32_t h, const char* path);
+/* Append one result row to a CSV file, writing the header first when the
+ * file does not exist.  GCQ_RT_CSV_QUEUE: the columns of rt_queue_perf.csv
+ * (rt_final.cpp:972-1001; every header column filled, GEN_MS = 0 because
+ * generation runs inside the trace launch).  GCQ_RT_CSV_COMPACT: the columns
+ * of rt_performance.csv (baselines/rt_compaction.cpp:849-864; THREADS is
+ * the block size). */
+enum gcq_rt_csv_style { GCQ_RT_CSV_QUEUE = 0, GCQ_RT_CSV_COMPACT = 1 };
+typedef struct gcq_rt_csv_row {
+  const char* gpu;         /* device name */
+  const char* queue;       /* "GWFQ", "GLFQ", "WFQ", "SFQ" or "COMPACT" */
+  const char* scene;       /* "complex", "cornell", ... (queue style only) */
+  uint32_t threads;        /* threads per tile (queue style) or block (compaction style) */
+  int32_t bounces;         /* max_depth (queue style only) */
+  const gcq_rt_result* res;
+} gcq_rt_csv_row;
+int gcq_rt_append_csv(const char* path, int32_t style, const gcq_rt_csv_row* row);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference.  A continuously batched server: BFS

@@ -36,6 +36,13 @@ class TbRes(C.Structure):
 _lib = None
 
 
+def _rt_sig(lib):
+    lib.orc_rt_render.restype = C.c_int
+    lib.orc_rt_render.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_int32, C.c_void_p, C.c_void_p]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -59,10 +66,7 @@ def load():
                                        C.POINTER(C.c_uint64)]
         lib.orc_bfs_parallel.restype = C.c_long
         lib.orc_bfs_parallel.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int]
-        lib.orc_rt_render.restype = C.c_int
-        lib.orc_rt_render.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                                      C.c_int32, C.c_void_p, C.c_void_p]
+        _rt_sig(lib)
         _lib = lib
     return _lib
 
@@ -134,7 +138,19 @@ def bfs_parallel(offsets, indices, src, threads=0):
     return depth, reached
 
 
-def rt_render(spheres, planes, cam12, light3, ambient3, W, H, x0=0, y0=0, w=0, h=0, max_depth=4, threads=0):
+def build_rt_variant(out_path, *defines):
+    """Compile rt_cpu.cpp alone with extra -D flags (planted-mistake copies for
+    tests) and return the loaded library, usable as rt_render(..., lib=...)."""
+    flags = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-shared"]
+    subprocess.run(["g++", *flags, *[f"-D{d}" for d in defines], os.path.join(HERE, "rt_cpu.cpp"), "-o",
+                    str(out_path)], check=True)
+    lib = C.CDLL(str(out_path))
+    _rt_sig(lib)
+    return lib
+
+
+def rt_render(spheres, planes, cam12, light3, ambient3, W, H, x0=0, y0=0, w=0, h=0, max_depth=4, threads=0,
+              lib=None):
     """CPU restatement of the reference ray tracer; returns (image[h, w, 3], rays traced)."""
     w = w or W
     h = h or H
@@ -145,7 +161,7 @@ def rt_render(spheres, planes, cam12, light3, ambient3, W, H, x0=0, y0=0, w=0, h
     am = np.ascontiguousarray(ambient3, np.float32)
     img = np.empty((h, w, 3), np.float32)
     rays = C.c_uint64(0)
-    rc = load().orc_rt_render(sph.ctypes.data, len(sph), pl.ctypes.data, len(pl), cam.ctypes.data, li.ctypes.data,
+    rc = (lib or load()).orc_rt_render(sph.ctypes.data, len(sph), pl.ctypes.data, len(pl), cam.ctypes.data, li.ctypes.data,
                               am.ctypes.data, W, H, x0, y0, w, h, max_depth, threads, img.ctypes.data, C.byref(rays))
     if rc != 0:
         raise ValueError("orc_rt_render: bad region")

@@ -7,6 +7,8 @@
 // produce because each pixel has exactly one live ray at a time.  Built
 // with -ffp-contract=off: every operation is the plain IEEE float
 // operation of the source expression, in source order.
+// -DPLANT_RT_ATTEN_NO_ALBEDO builds a copy with a planted mistake in the
+// reflection attenuation, which the closed-form tests must reject.
 #include <cmath>
 #include <cstdint>
 #include <thread>
@@ -128,7 +130,11 @@ extern "C" int orc_rt_render(const float* sph, int32_t ns, const float* pl, int3
             if (nr.x > 0.f || nr.y > 0.f || nr.z > 0.f) pix = add(pix, nr);
             o = add(hit.p, scale(hit.n, 1e-4f));
             d = normalize3(reflect3(d, hit.n));
+#ifdef PLANT_RT_ATTEN_NO_ALBEDO
+            atten = scale(atten, hit.refl);  // planted mistake: the albedo factor dropped
+#else
             atten = scale(mulv(atten, hit.albedo), hit.refl);
+#endif
             continue;
           }
           pix = add(pix, clamp3(contrib));  // :610-614

@@ -1,8 +1,12 @@
 """CPU tests pinning the ray-tracing oracle (a restatement of
 rt_final.cpp's image) against closed forms computed here in float64 from the
 geometry alone: the sky gradient of an empty scene, a perfect mirror floor
-(sky of the reflected direction), an analytic sphere hit, the affine
-dependence of a Lambertian image on albedo, and the ray count bound."""
+(sky of the reflected direction), a partial mirror with albedo != 1 at bounce
+limits 0, 1 and 2 (the split of rt_final.cpp:581-609), two facing partial
+mirrors (the attenuation compounded as (albedo * refl)^k), an analytic sphere
+hit, the affine dependence of a Lambertian image on albedo, and the ray count
+bound.  A copy of the oracle built with a planted mistake in the reflection
+attenuation (albedo dropped) must fail the reflective checks."""
 import numpy as np
 import pytest
 
@@ -28,12 +32,12 @@ def _sky(dy):
     return np.clip((1 - t)[..., None] + t[..., None] * SKY_HI, 0, 1)
 
 
-def _render(orc, spheres, planes, W=160, H=90, cam_o=(0, 0.3, 0.5), cam_f=(0, -0.2, -1), max_depth=4):
+def _render(orc, spheres, planes, W=160, H=90, cam_o=(0, 0.3, 0.5), cam_f=(0, -0.2, -1), max_depth=4, lib=None):
     cam = np.array(list(cam_o) + list(cam_f) + [1, 0, 0, 0, 1, 0], np.float32)
     sph = np.asarray(spheres, np.float32).reshape(-1, 8)
     pl = np.asarray(planes, np.float32).reshape(-1, 8)
     return orc.rt_render(sph, pl, cam, LIGHT.astype(np.float32), np.full(3, AMB, np.float32), W, H,
-                         max_depth=max_depth, threads=2)
+                         max_depth=max_depth, threads=2, lib=lib)
 
 
 def test_empty_scene_is_sky(orc):
@@ -126,3 +130,97 @@ def test_presets_match_reference_scene_sizes(gcq):
         assert np.allclose(sph[:, 1], -1 + sph[:, 3])  # resting on the plane y = -1
         mirrors = np.sum(sph[:, 7] > 0)
         assert 8 <= mirrors <= 32 and set(np.unique(sph[:, 7]).tolist()) <= {0.0, float(np.float32(0.8))}
+
+
+# ---------------------------------------------------------------- partial mirrors
+# rt_final.cpp:581-614: a hit adds clamp01(contrib * (1 - refl)) and sends the
+# reflected ray on with atten * albedo * refl while depth < max_depth;
+# otherwise it adds clamp01(contrib) and ends.  contrib = (ambient + albedo *
+# max(0, n.L)) * atten.
+ALB = np.array([0.6, 0.5, 0.4])
+REFL = 0.35
+NL_UP = LIGHT[1] / np.linalg.norm(LIGHT)  # n.L for the upward normal (0, 1, 0)
+
+
+def _partial_floor_expect(d, max_depth, alb=ALB, refl=REFL):
+    dy = d[..., 1]
+    lam = AMB + alb * NL_UP
+    if max_depth == 0:
+        hit = np.broadcast_to(np.clip(lam, 0, 1), d.shape)
+    else:  # the reflected ray (d.x, -d.y, d.z) leaves the floor and sees the sky
+        hit = np.clip(lam * (1 - refl), 0, 1) + np.clip(_sky(-dy) * alb * refl, 0, 1)
+    return np.where((dy < 0)[..., None], hit, _sky(dy))
+
+
+def _partial_floor_err(render, max_depth):
+    floor = [0, 1, 0, 1.0, *ALB, REFL]
+    img, rays = render([], [floor], max_depth=max_depth)
+    d = _camera_dirs(160, 90)
+    ok = np.abs(d[..., 1]) > 1e-3
+    err = np.abs(img - _partial_floor_expect(d, max_depth))[ok].max()
+    down = int((d[..., 1] < 0).sum())
+    return err, rays, down
+
+
+@pytest.mark.parametrize("max_depth", [0, 1, 2])
+def test_partial_mirror_floor(orc, max_depth):
+    err, rays, down = _partial_floor_err(lambda *a, **k: _render(orc, *a, **k), max_depth)
+    assert err < 3e-6, err
+    # one bounce at most: the reflected ray escapes, so depth 2 equals depth 1
+    assert rays == 160 * 90 + (down if max_depth >= 1 else 0)
+
+
+# Two facing partial mirrors, floor y = -1 (normal up) and ceiling y = +1
+# (normal down, n.L < 0 so only ambient), camera between them: every ray that
+# is not near-horizontal bounces between them until max_depth.  The k-th hit
+# adds clamp01((ambient + alb * nl_k) * (alb * refl)^k * (1 - refl)), the
+# last one clamp01((ambient + alb * nl_m) * (alb * refl)^m).
+ALB2 = np.array([0.9, 0.7, 0.5])
+REFL2 = 0.6
+
+
+def _two_mirrors_expect(d, max_depth):
+    dy = d[..., 1]
+    out = np.zeros(d.shape)
+    for k in range(max_depth + 1):
+        on_floor = (dy < 0) if k % 2 == 0 else (dy > 0)  # the first hit is the plane the ray points at
+        nl = np.where(on_floor, NL_UP, 0.0)[..., None]
+        contrib = (AMB + ALB2 * nl) * (ALB2 * REFL2) ** k
+        out += np.clip(contrib * ((1 - REFL2) if k < max_depth else 1.0), 0, 1)
+    return out
+
+
+def _two_mirrors_err(render, max_depth=4):
+    floor = [0, 1, 0, 1.0, *ALB2, REFL2]
+    ceiling = [0, -1, 0, 1.0, *ALB2, REFL2]
+    img, rays = render([], [floor, ceiling], cam_o=(0, 0, 0.5), cam_f=(0, -0.2, -1), max_depth=max_depth)
+    d = _camera_dirs(160, 90)
+    ok = np.abs(d[..., 1]) > 0.05  # hit points close enough that the 1e-4 offset clears the plane
+    err = np.abs(img - _two_mirrors_expect(d, max_depth))[ok].max()
+    return err, rays, int(ok.sum())
+
+
+def test_two_facing_mirrors(orc):
+    err, rays, n_ok = _two_mirrors_err(lambda *a, **k: _render(orc, *a, **k))
+    assert n_ok > 5000
+    assert err < 5e-6, err
+    # every non-horizontal pixel traces max_depth + 1 rays
+    assert rays >= n_ok * 5
+
+
+def test_planted_attenuation_mistake_is_caught(orc, tmp_path):
+    """The same checks against a copy of the oracle whose reflection ray
+    carries atten * refl (albedo dropped) must fail: the partial floor at one
+    bounce, and the compounded attenuation of the facing mirrors."""
+    bad = orc.build_rt_variant(tmp_path / "librt_bad.so", "PLANT_RT_ATTEN_NO_ALBEDO")
+
+    def render(*a, **k):
+        return _render(orc, *a, lib=bad, **k)
+
+    err1, _, _ = _partial_floor_err(render, 1)
+    err2, _, _ = _two_mirrors_err(render)
+    assert err1 > 1e-3, err1
+    assert err2 > 1e-3, err2
+    # depth 0 has no reflection ray, so it cannot tell the two apart
+    err0, _, _ = _partial_floor_err(render, 0)
+    assert err0 < 3e-6
