@@ -56,7 +56,8 @@ class TbConfig(C.Structure):
 class TbResult(C.Structure):
     _fields_ = [("enq_success", C.c_uint64), ("enq_fail", C.c_uint64), ("deq_success", C.c_uint64),
                 ("deq_empty", C.c_uint64), ("launches", C.c_uint64), ("elapsed_ms", C.c_double),
-                ("kernel_ms_mean", C.c_double), ("succ_mops", C.c_double)]
+                ("kernel_ms_mean", C.c_double), ("succ_mops", C.c_double), ("pre_enq_success", C.c_uint64),
+                ("pre_deq_success", C.c_uint64)]
 
 
 class FifoResult(C.Structure):
@@ -109,6 +110,7 @@ def load():
         "gcq_hbm_copy": (C.c_int, [C.c_uint64, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
         "gcq_dropin_selfcheck": (C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, _P(C.c_uint32)]),
         "gcq_tb_run": (C.c_int, [C.c_void_p, _P(TbConfig), _P(TbResult)]),
+        "gcq_queue_drain": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint64)]),
         "gcq_fifo_check": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _P(FifoResult)]),
         "gcq_hist_run": (C.c_int, [C.c_void_p, _P(HistConfig), C.c_void_p, C.c_size_t, _P(C.c_size_t)]),
         "gcq_hist_write_jsonl": (C.c_int, [C.c_void_p, C.c_size_t, C.c_char_p]),
@@ -211,6 +213,12 @@ class Queue:
         res = TbResult()
         _check(load().gcq_tb_run(self.handle, C.byref(cfg), C.byref(res)), "gcq_tb_run")
         return {f: getattr(res, f) for f, _ in TbResult._fields_}
+
+    def drain(self, threads=4096):
+        """Dequeue everything the (quiescent) queue holds; returns the count."""
+        n = C.c_uint64(0)
+        _check(load().gcq_queue_drain(self.handle, threads, C.byref(n)), "gcq_queue_drain")
+        return n.value
 
     def fifo_check(self, threads, ops, block=256, concurrent=False):
         res = FifoResult()

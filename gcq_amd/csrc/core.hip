@@ -502,6 +502,17 @@ extern "C" int gcq_tb_run(gcq_queue* q, const gcq_tb_config* cfg, gcq_tb_result*
         if (per_sync == 0) {
           per_sync = (uint32_t)std::max(1.0, std::min(256.0, 5.0 / std::max(res->kernel_ms_mean, 1e-3)));
         }
+        // Counts of the warmup and probe launches, so that a caller can
+        // account for every item the run put in or took out.
+        {
+          std::vector<uint64_t> pre(4ull * grid);
+          GCQ_HIP(hipMemcpyAsync(pre.data(), stats, 32ull * grid, hipMemcpyDeviceToHost, q->stream));
+          GCQ_HIP(hipStreamSynchronize(q->stream));
+          for (uint32_t b = 0; b < grid; ++b) {
+            res->pre_enq_success += pre[4 * b + 0];
+            res->pre_deq_success += pre[4 * b + 2];
+          }
+        }
         GCQ_HIP(hipMemsetAsync(stats, 0, 32ull * grid, q->stream));
         uint64_t launches = 0;
         if (cfg->timed_launches > 0) {
@@ -544,6 +555,50 @@ extern "C" int gcq_tb_run(gcq_queue* q, const gcq_tb_config* cfg, gcq_tb_result*
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(stats);
+  return rc;
+}
+
+// ======================================================================== drain
+namespace {
+
+// Every lane dequeues until its dequeue answers EMPTY; on a quiescent queue
+// the first EMPTY means the queue is empty (each queue's EMPTY answer is
+// linearizable, and nothing is enqueued meanwhile).
+template <class Q>
+__global__ void k_drain(Q q, uint32_t threads, uint64_t* count) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t n = 0;
+  if (tid < threads) {
+    uint64_t v;
+    while (q.deq(tid, &v)) ++n;
+  }
+  n = gcq::wave_sum(n);
+  if (gcq::lane_id() == 0 && n) gcq::faa_rlx(count, n);
+}
+
+}  // namespace
+
+extern "C" int gcq_queue_drain(gcq_queue* q, uint32_t threads, uint64_t* items) {
+  GCQ_REQUIRE(q && items, "null argument");
+  GCQ_REQUIRE(threads >= 64 && threads % 64 == 0 && threads <= q->cfg.max_threads,
+              "threads must be a multiple of 64 in [64, max_threads]");
+  uint64_t* d = nullptr;
+  GCQ_HIP(hipMalloc(&d, 8));
+  GCQ_HIP(hipMemsetAsync(d, 0, 8, q->stream));
+  int rc = gcq_dispatch(q, [&](auto a) -> int {
+    using A = decltype(a);
+    hipLaunchKernelGGL(k_drain<A>, dim3(threads / 64), dim3(64), 0, q->stream, a, threads, d);
+    GCQ_HIP(hipGetLastError());
+    return GCQ_OK;
+  });
+  if (rc == GCQ_OK) {
+    if (hipMemcpyAsync(items, d, 8, hipMemcpyDeviceToHost, q->stream) != hipSuccess ||
+        hipStreamSynchronize(q->stream) != hipSuccess) {
+      gcqh::set_error("drain readback failed");
+      rc = GCQ_EHIP;
+    }
+  }
+  (void)hipFree(d);
   return rc;
 }
 

@@ -127,11 +127,20 @@ typedef struct gcq_tb_result {
   double elapsed_ms;        /* event time over the timed window (ref semantics) */
   double kernel_ms_mean;    /* mean duration of one chunk launch */
   double succ_mops;         /* (enq_success + deq_success) / elapsed */
+  uint64_t pre_enq_success; /* successes in the untimed warmup and probe launches */
+  uint64_t pre_deq_success;
 } gcq_tb_result;
 /* Timed balanced / split run.  Replaces run_balanced_timed /
  * run_split_timed + balanced_chunk_kernel / split_chunk_kernel
  * (throughout_bench.cpp:810-882, 959-1051). */
 int gcq_tb_run(gcq_queue* q, const gcq_tb_config* cfg, gcq_tb_result* res);
+
+/* Dequeue everything a quiescent queue holds (no operation in flight) and
+ * return the item count in *items, so that a caller can check conservation
+ * after a timed run: enqueued = dequeued + drained.  No reference
+ * counterpart (the reference's harnesses never account for leftovers,
+ * throughout_bench.cpp:959-1051). */
+int gcq_queue_drain(gcq_queue* q, uint32_t threads, uint64_t* items);
 
 /* ------------------------------------------------------------------ FIFO check */
 typedef struct gcq_fifo_result {

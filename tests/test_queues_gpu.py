@@ -76,18 +76,30 @@ def test_fifo_check_concurrent_65536(gcq, kind):
     assert r["verified"], r
 
 
+def _conserved(q, r, what):
+    """Every item enqueued over the run (warmup, probe and timed launches) was
+    dequeued in it or is still in the queue: drain the quiescent queue and
+    count exactly."""
+    left = q.drain(4096)
+    enq = r["pre_enq_success"] + r["enq_success"]
+    deq = r["pre_deq_success"] + r["deq_success"]
+    print(f"[{what}] enq {enq} deq {deq} left {left}", flush=True)
+    assert enq == deq + left, (enq, deq, left)
+    assert q.drain(64) == 0
+
+
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
 @pytest.mark.parametrize("solo", [False, True], ids=["combined", "solo"])
-def test_throughput_conserves_items(gcq, kind, solo):
+@pytest.mark.parametrize("mode", [1, 0], ids=["split25", "balanced"])
+def test_throughput_conserves_items(gcq, kind, solo, mode):
     # 8,192 threads: the smallest grid in which the harness combines claims
-    print(f"[tb] kind={NAMES[kind]} solo={solo}", flush=True)
+    print(f"[tb] kind={NAMES[kind]} solo={solo} mode={mode}", flush=True)
     with _queue(gcq, kind, 8192, capacity=1 << 16) as q:
-        r = q.tb_run(1, threads=8192, chunk_ops=32, producer_percent=25, warmup_ms=20, run_ms=100, solo_claims=solo)
-    assert r["launches"] > 0 and r["elapsed_ms"] > 0
-    assert r["enq_success"] > 0 and r["deq_success"] > 0
-    # items dequeued in the window can exceed those enqueued in it only by
-    # what was queued before the window (at most the capacity)
-    assert r["deq_success"] <= r["enq_success"] + (1 << 16) + 8192 * 32
+        r = q.tb_run(mode, threads=8192, chunk_ops=32, producer_percent=25, warmup_launches=3, timed_launches=20,
+                     solo_claims=solo)
+        assert r["launches"] == 20 and r["elapsed_ms"] > 0
+        assert r["enq_success"] > 0 and r["deq_success"] > 0
+        _conserved(q, r, f"tb {NAMES[kind]}")
 
 
 @pytest.mark.parametrize("kind", [0, 1, 2], ids=["gwfq", "glfq", "wfq"])
@@ -99,10 +111,12 @@ def test_grouped_tail_tickets_conserve_items(gcq, kind, mode):
     kw = {"wfq_segment_cells": 1 << 26} if kind == 2 else {}
     with gcq.Queue(kind, capacity=1 << 20, max_threads=32768, poll_backoff=0, **kw) as q:
         r = q.tb_run(mode, threads=32768, chunk_ops=32, producer_percent=25, warmup_launches=2, timed_launches=5)
-    print(f"[tb grouped] succ {r['succ_mops']:.1f} Mops/s, enq {r['enq_success']} deq {r['deq_success']}", flush=True)
-    assert r["enq_fail"] == 0
-    assert r["enq_success"] == (32768 // 4 if mode == 1 else 32768) * 32 * 5
-    assert 0 < r["deq_success"] <= r["enq_success"] + 32768 * 32 * 2
+        print(f"[tb grouped] succ {r['succ_mops']:.1f} Mops/s, enq {r['enq_success']} deq {r['deq_success']}",
+              flush=True)
+        assert r["enq_fail"] == 0
+        assert r["enq_success"] == (32768 // 4 if mode == 1 else 32768) * 32 * 5
+        assert r["deq_success"] > 0
+        _conserved(q, r, f"tb grouped {NAMES[kind]}")
 
 
 @pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
@@ -181,6 +195,6 @@ def test_full_queue_keeps_moving(gcq, kind):
         print(f"[full queue] kind={NAMES[kind]} pp={pp}", flush=True)
         with gcq.Queue(kind, capacity=65536, max_threads=65536, poll_backoff=0) as q:
             r = q.tb_run(1, threads=16384, chunk_ops=128, producer_percent=pp, warmup_launches=5, timed_launches=20)
-        print(f"[full queue] succ {r['succ_mops']:.1f} Mops/s, launch {r['kernel_ms_mean']:.2f} ms", flush=True)
-        assert r["deq_success"] <= r["enq_success"] + 65536
+            print(f"[full queue] succ {r['succ_mops']:.1f} Mops/s, launch {r['kernel_ms_mean']:.2f} ms", flush=True)
+            _conserved(q, r, f"full queue {NAMES[kind]} pp={pp}")
         assert r["kernel_ms_mean"] < 25.0
