@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <type_traits>
@@ -477,6 +478,105 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   }
   if (h.err) {
     gcqh::set_error("BFS frontier enqueue failed (queue capacity or cell pool exhausted)");
+    return GCQ_EFULL;
+  }
+  return GCQ_OK;
+}
+
+// ======================================================================== allocation-free BFS
+// A BFS over a caller's device-resident CSR into a caller's device depth
+// array that allocates nothing: the two G-LFQ frontier rings and the control
+// block live in `scratch`, a file-scope device array each GPU holds until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in host
+// code the name stands for a host-side shadow, not a device address); it
+// uses the per-device address hipGetSymbolAddress returned.  The kernels are
+// the ones gcq_bfs_run launches for G-LFQ frontiers in persistent mode.
+namespace {
+
+constexpr int32_t kNoAllocMaxVertices = 1 << 21;
+constexpr uint64_t kNoAllocRingBytes = gcq::lfq::ring_bytes((uint32_t)kNoAllocMaxVertices);
+constexpr uint64_t kNoAllocCtlOff = 2 * kNoAllocRingBytes;
+constexpr uint64_t kNoAllocBytes = kNoAllocCtlOff + ((sizeof(bfsk::Ctl) + 255) & ~255ull);
+static_assert(kNoAllocRingBytes % 256 == 0, "ring alignment");
+static_assert(kNoAllocBytes == GCQ_BFS_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch array, looked up once per
+// device (with that device current).
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  GCQ_HIP(hipGetDevice(&dev));
+  GCQ_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d devices supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    GCQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GCQ_REQUIRE(p != nullptr, "no device address for the BFS scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GCQ_OK;
+}
+
+}  // namespace
+
+extern "C" int gcq_bfs_run_noalloc(const int32_t* d_offsets, const int32_t* d_indices, int32_t nv, int64_t ne,
+                                   int32_t src, uint32_t threads, uint32_t block, void* stream, int32_t* d_depth,
+                                   gcq_bfs_result* res) {
+  GCQ_REQUIRE(d_offsets && d_depth && res && (d_indices || ne == 0), "null argument");
+  GCQ_REQUIRE(nv >= 1 && nv <= kNoAllocMaxVertices,
+              "nv = %d: the allocation-free BFS takes 1 to %d vertices (its frontier rings are fixed)", nv,
+              kNoAllocMaxVertices);
+  GCQ_REQUIRE(ne >= 0 && ne < (1ll << 31), "ne out of range");
+  GCQ_REQUIRE(block >= 64 && block % 64 == 0 && block <= 1024, "block must be a multiple of 64");
+  GCQ_REQUIRE(threads >= block && threads % block == 0 && threads <= (uint32_t)kNoAllocMaxVertices,
+              "threads must be a positive multiple of block, at most %d", kNoAllocMaxVertices);
+  std::memset(res, 0, sizeof *res);
+  if (src < 0 || src >= nv) src = 0;  // bfs_bench.cpp:555
+  void* base = nullptr;
+  int rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  int dev = 0, per_cu = 0, cus = 0;
+  GCQ_HIP(hipGetDevice(&dev));
+  GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfsk::k_persistent<gcqf::FLfq>, block, 0));
+  const uint32_t grid = threads / block;
+  GCQ_REQUIRE((int64_t)grid <= (int64_t)per_cu * cus, "persistent BFS needs all %u blocks resident; the device holds %d",
+              grid, per_cu * cus);
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t cap = gcqf::pow2_at_least(std::max<uint64_t>((uint64_t)nv, threads));
+  void* ring0 = base;
+  void* ring1 = (char*)base + kNoAllocRingBytes;
+  bfsk::Ctl* ctl = (bfsk::Ctl*)((char*)base + kNoAllocCtlOff);
+  bfsk::Frontiers<gcqf::FLfq> fq;
+  for (int i = 0; i < 3; ++i) fq.q[i] = gcqf::FLfq{(gcq::lfq::Ring*)(i % 2 ? ring1 : ring0)};
+  const auto t0 = std::chrono::steady_clock::now();
+  hipLaunchKernelGGL(gcqf::kf_fill_i32, dim3(gcqf::grid_of(nv)), dim3(256), 0, s, d_depth, (uint64_t)nv, -1);
+  hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, s, (uint64_t*)ctl, (uint64_t)(sizeof(bfsk::Ctl) / 8));
+  hipLaunchKernelGGL(gcqf::kf_reset_lfq, dim3(gcqf::grid_of(2ull * cap)), dim3(256), 0, s, ring0, ring1, cap);
+  hipLaunchKernelGGL(bfsk::k_seed<gcqf::FLfq>, dim3(1), dim3(64), 0, s, fq.q[0], d_depth, src, ctl);
+  hipLaunchKernelGGL(bfsk::k_persistent<gcqf::FLfq>, dim3(grid), dim3(block), 0, s, fq, d_offsets, d_indices, d_depth,
+                     ctl);
+  GCQ_HIP(hipGetLastError());
+  bfsk::Ctl h;
+  GCQ_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, s));
+  GCQ_HIP(hipStreamSynchronize(s));
+  res->edges_scanned = h.edges;
+  res->reached = h.reached + 1;
+  res->levels = h.levels;
+  res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  res->kernel_ms = res->total_ms;  // no events: they would be allocations
+  if (h.lbar.abort) {
+    gcqh::set_error("BFS aborted: a grid-barrier wait exceeded its spin limit");
+    return GCQ_ETIMEOUT;
+  }
+  if (h.err) {
+    gcqh::set_error("BFS frontier enqueue failed");
     return GCQ_EFULL;
   }
   return GCQ_OK;

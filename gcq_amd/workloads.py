@@ -86,6 +86,8 @@ SIGNATURES = {
     "gcq_graph_pick_source": (C.c_int32, [VP, C.c_uint64, C.c_uint64]),
     "gcq_graph_destroy": (C.c_int, [VP]),
     "gcq_bfs_run": (C.c_int, [VP, C.c_int32, _P(BfsConfig), VP, _P(BfsResult)]),
+    "gcq_bfs_run_noalloc": (C.c_int, [VP, VP, C.c_int32, C.c_int64, C.c_int32, C.c_uint32, C.c_uint32, VP, VP,
+                                      _P(BfsResult)]),
     "gcq_batch_create": (C.c_int, [_P(VP), C.c_int32, _P(VP)]),
     "gcq_batch_config_default": (None, [_P(BatchConfig)]),
     "gcq_batch_run": (C.c_int, [VP, _P(BatchConfig), C.c_int32, VP, VP, _P(VP), _P(QueryResult), _P(BatchResult)]),

@@ -235,6 +235,25 @@ typedef struct gcq_bfs_result {
  * bfs_expand_kernel + bfs_hub_kernel (bfs_bench.cpp:327-633). */
 int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg, int32_t* depth_out, gcq_bfs_result* res);
 
+/* Allocation-free BFS.  No counterpart in the reference: this entry point is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * gcq_bfs_run's persistent kernel with G-LFQ frontiers over a CSR the caller
+ * holds in device memory (d_offsets[nv + 1], d_indices[ne], sorted,
+ * symmetric) and writes nv int32 depths (-1 = unreached) to the caller's
+ * device buffer d_depth.  It calls no hipMalloc/hipFree: its frontier rings
+ * and control block live in a fixed device array of
+ * GCQ_BFS_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends.  Limits: 1 <= nv <= 2^21 vertices and threads <= 2^21
+ * (GCQ_EINVAL beyond).  stream is a hipStream_t (NULL = the default stream);
+ * the call waits for the traversal and fills res (kernel_ms is host wall time
+ * around the launch sequence).  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU race; the array keeps the last
+ * call's contents and each call re-initialises what it uses. */
+#define GCQ_BFS_NOALLOC_SCRATCH_BYTES 67113984ull
+int gcq_bfs_run_noalloc(const int32_t* d_offsets, const int32_t* d_indices, int32_t nv, int64_t ne, int32_t src,
+                        uint32_t threads, uint32_t block, void* stream, int32_t* d_depth, gcq_bfs_result* res);
+
 /* ------------------------------------------------------------------ batched BFS
  * No counterpart in the reference (its BFS harness runs one query per
  * process, bfs_bench.cpp:653-800).  Many traversals, over any of a set of
