@@ -198,3 +198,18 @@ def test_full_queue_keeps_moving(gcq, kind):
             print(f"[full queue] succ {r['succ_mops']:.1f} Mops/s, launch {r['kernel_ms_mean']:.2f} ms", flush=True)
             _conserved(q, r, f"full queue {NAMES[kind]} pp={pp}")
         assert r["kernel_ms_mean"] < 25.0
+
+
+@pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
+def test_remap_history_and_conservation(gcq, orc, kind):
+    # remap = 1 spreads consecutive tickets over 128-B lines (cache_remap,
+    # gwf_queue.hpp:218-230, glf_queue.hpp:116-120).  A 1,024-item ring under
+    # 1,024 threads x 16 ops turns many laps through the permuted slots.
+    print(f"[remap] kind={NAMES[kind]}", flush=True)
+    with _queue(gcq, kind, 1024, capacity=1024, remap=1) as q:
+        h = q.hist_run(threads=1024, ops=16, block=256, mode=0)
+    _check(orc, h, f"{NAMES[kind]} remap")
+    with gcq.Queue(kind, capacity=1 << 16, max_threads=8192, remap=1) as q:
+        r = q.tb_run(1, threads=8192, chunk_ops=32, producer_percent=25, warmup_launches=3, timed_launches=20)
+        assert r["enq_success"] > 0 and r["deq_success"] > 0
+        _conserved(q, r, f"remap {NAMES[kind]}")
