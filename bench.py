@@ -41,6 +41,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The OpenMP BFS baseline binds its threads close together (set before the
+# oracle library first starts OpenMP).
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one successful operation through the G-WFQ
@@ -64,14 +67,29 @@ def _profile_traffic(name):
 QUEUE_NAMES = {0: "G-WFQ", 1: "G-LFQ", 2: "WFQ", 3: "SFQ"}
 
 
-def cpu_baseline(kind, threads, run_ms, warmup_ms=200):
+def host_cores():
+    """The host share of one GPU is 16 cores, whatever os.cpu_count() reports."""
+    return min(len(os.sched_getaffinity(0)), 16)
+
+
+def cpu_baseline(kind, cores, run_ms, warmup_ms=200):
+    """The oracle's host std::atomic restatement of the queue, split 25/75,
+    swept over 1, 2, 4, ... host threads up to the host share: value is the
+    best rate (cores = the threads that gave it), beside the all-cores rate."""
     from oracle import pyoracle as orc
-    r = orc.tb_run(kind, threads=threads, mode=1, producer_percent=25, chunk_ops=128, warmup_ms=warmup_ms,
-                   run_ms=run_ms, capacity=1 << 20)
-    return {"value": round(r["succ_mops"], 3), "unit": "Mops/s", "cores": threads, "kind": "port",
-            "sample": f"host std::atomic {QUEUE_NAMES[kind]} restatement, split 25/75, {threads} threads, chunk 128, "
-                      f"{warmup_ms} ms warmup + {run_ms} ms timed, capacity 2^20 (hardware_concurrency="
-                      f"{r['hw_threads']})"}
+    counts = sorted({1 << i for i in range(cores.bit_length()) if (1 << i) <= cores} | {cores})
+    sweep = {}
+    for n in counts:
+        r = orc.tb_run(kind, threads=n, mode=1, producer_percent=25, chunk_ops=128, warmup_ms=warmup_ms,
+                       run_ms=run_ms, capacity=1 << 20)
+        sweep[n] = r["succ_mops"]
+    best = max(sweep, key=sweep.get)
+    return {"value": round(sweep[best], 3), "unit": "Mops/s", "cores": best, "kind": "port",
+            "all_cores": {"value": round(sweep[cores], 3), "cores": cores},
+            "sweep": {str(n): round(v, 3) for n, v in sweep.items()},
+            "sample": f"host std::atomic {QUEUE_NAMES[kind]} restatement, split 25/75, swept over "
+                      f"{', '.join(map(str, counts))} threads, chunk 128, {warmup_ms} ms warmup + {run_ms} ms timed "
+                      f"each, capacity 2^20 (hardware_concurrency={r['hw_threads']}, host share {cores})"}
 
 
 def main():
@@ -80,18 +98,30 @@ def main():
     ap.add_argument("--steps", type=int, default=120, help="timed chunk launches")
     ap.add_argument("--warmup", type=int, default=25, help="untimed chunk launches")
     ap.add_argument("--threads", type=int, default=65536)
-    ap.add_argument("--queue", default="gwfq", choices=["gwfq", "glfq", "wfq", "sfq"])
+    ap.add_argument("--queue", default=None, choices=["gwfq", "glfq", "wfq", "sfq"],
+                    help="queue kind (default gwfq; --profile bfs: glfq)")
     ap.add_argument("--cpu-baseline-ms", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the BFS / batched / served / RT lines")
     ap.add_argument("--no-solo", action="store_true",
                     help="skip the solo-claims comparison run (profiles of the headline kernel alone)")
+    ap.add_argument("--profile", choices=PROFILE_TARGETS,
+                    help="run one workload only and print its result, for rocprofv3 kernel traces and HBM counters")
+    ap.add_argument("--runs", type=int, default=3, help="--profile: repetitions of the workload")
+    ap.add_argument("--bytes", type=int, default=1 << 30, help="--profile copy: bytes copied")
+    ap.add_argument("--iters", type=int, default=5, help="--profile copy: launches")
+    ap.add_argument("--mode", default="persistent", help="--profile bfs: BFS mode")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(each queue lives in one GPU's memory; several GPUs would each run their own queues)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.profile:
+        print(json.dumps(profile_run(args)), flush=True)
+        return
+    args.queue = args.queue or "gwfq"
 
     import gcq_amd as g
     kind = g.KINDS[args.queue]
@@ -136,10 +166,12 @@ def main():
                                                    solo_claims=True)
     succ = r["enq_success"] + r["deq_success"]
     value = r["succ_mops"]
+    # HIP events on the queue's stream bracket exactly the K timed launches,
+    # which run back to back: their mean is the launch duration.
     ms_per_step = r["elapsed_ms"] / args.steps
     succ_per_launch = succ / args.steps
     kernel_ms = r["kernel_ms_mean"]
-    achieved_ops = succ_per_launch / (kernel_ms * 1e-3)
+    achieved_ops = succ_per_launch / (ms_per_step * 1e-3)
 
     line = {
         "metric": "split-role queue Mops/s",
@@ -164,6 +196,10 @@ def main():
                      "peak": round(p1["tickets_per_s"] / 1e9, 4), "unit": "Gop/s",
                      "frac": round(achieved_ops / p1["tickets_per_s"], 4),
                      "traffic": _profile_traffic("tb_split_gwfq_65536.json"),
+                     "traffic_source": "profiles/tb_split_gwfq_65536.json: rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                                       "launch of this kernel, scaled by a copy kernel (committed, not measured "
+                                       "in this run)",
+                     "duration": "mean of the timed launches (HIP events on the launch stream)",
                      "peak_kind": "single-counter wave-aggregated FAA (tickets/s), measured in this run",
                      "peak_striped4_gops": round(p4["tickets_per_s"] / 1e9, 3)},
         "roofline_hbm": {"bound": "hbm", "achieved": round(achieved_ops * BYTES_PER_OP / 1e9, 3),
@@ -178,12 +214,79 @@ def main():
                    "elapsed_ms": round(r["elapsed_ms"], 3), "device": info["name"], "cus": info["cus"]},
     }
     if not args.no_cpu_baseline:
-        # the host share of one GPU is 16 cores whatever os.cpu_count() reports
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        line["cpu_baseline"] = cpu_baseline(kind, cores, args.cpu_baseline_ms)
+        line["cpu_baseline"] = cpu_baseline(kind, host_cores(), args.cpu_baseline_ms)
     if not args.headline_only:
         line["secondary"] = secondary(g, not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
+
+
+PROFILE_TARGETS = ["copy", "tb", "tb-balanced", "bfs", "batch", "rt", "rtc", "serve"]
+
+
+def profile_run(a):
+    """One workload, --runs times, for rocprofv3 (profiles/ give the commands):
+      copy         streaming copy of --bytes, --iters launches (calibrates HBM byte counters)
+      tb           the headline: split 25/75, 65,536 threads, 128 ops, capacity 2^20, --queue,
+                   --warmup + --steps launches
+      tb-balanced  the same grid in balanced mode
+      bfs          the 4900 x 4900 lattice, G-LFQ, 8,192 threads, --mode
+      batch        256 R-MAT queries (scales 12..20) in one batched launch
+      rt / rtc     the 1080p 64-sphere frame, queue renderer / stream compaction
+      serve        the served line's burst: 300 requests at once"""
+    import gcq_amd as g
+    out = {"what": a.profile}
+    if a.profile == "copy":
+        gbps, ms = g.hbm_copy(a.bytes, a.iters)
+        out.update(bytes=a.bytes, gbps=gbps, kernel_ms=ms)
+    elif a.profile in ("tb", "tb-balanced"):
+        p1 = g.faa_peak(threads=65536, block=256, counters=1, stride_bytes=128, aggregate=True, iters=64, launches=10)
+        with g.Queue(g.KINDS[a.queue or "gwfq"], capacity=1 << 20, max_threads=65536, poll_backoff=0) as q:
+            r = q.tb_run(g.SPLIT if a.profile == "tb" else 0, threads=65536, block=256, chunk_ops=128,
+                         producer_percent=25, warmup_launches=a.warmup, timed_launches=a.steps)
+        ms = r["elapsed_ms"] / a.steps
+        out.update(r, ms_per_step=ms, p1_tickets_per_s=p1["tickets_per_s"],
+                   frac_p1=(r["enq_success"] + r["deq_success"]) / a.steps / (ms * 1e-3) / p1["tickets_per_s"])
+    elif a.profile == "bfs":
+        from gcq_amd import bfs
+        mode = {"persistent": bfs.PERSISTENT, "host": bfs.HOST_LOOP}.get(a.mode, a.mode)
+        with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as gr:
+            for _ in range(a.runs):
+                _, res = gr.bfs(0, kind=g.KINDS[a.queue or "glfq"], threads=8192, block=256, mode=mode, depths=False)
+        out.update(res)
+    elif a.profile == "batch":
+        from gcq_amd import bfs
+        graphs, gids, srcs = bfs.rmat_query_set()
+        with bfs.Batch(graphs) as b:
+            for _ in range(a.runs):
+                _, _, res = b.run(gids, srcs, threads=262144, block=512, use_graph=0)
+        out.update(res)
+        for gr in graphs:
+            gr.close()
+    elif a.profile in ("rt", "rtc"):
+        from gcq_amd import rt
+        with rt.Scene.preset(rt.RANDOM, 64, 3) as sc:
+            for _ in range(a.runs):
+                if a.profile == "rt":
+                    _, res = sc.render(width=1920, height=1080, tile_w=256, tile_h=256, threads_per_tile=8192,
+                                       block=512)
+                else:
+                    _, res = sc.render_compact(width=1920, height=1080, block=1024)
+        out.update(res)
+    elif a.profile == "serve":
+        from gcq_amd import bfs, rt, serve
+        graphs = [bfs.Graph.rmat(sc, 16, 0.57, 0.19, 0.19, 11) for sc in range(14, 23)]
+        nvs = [gr.info()[0] for gr in graphs]
+        rows = serve.poisson_trace(300, 1e9, nvs, (1920, 1080), rt_fraction=0.1, seed=6,
+                                   graph_sources=lambda gi, i: graphs[gi].pick_source(11, i))
+        with rt.Scene.preset(rt.RANDOM, 64, 3) as scene:
+            for _ in range(a.runs):
+                with serve.Server(graphs, scene, (1920, 1080), bfs_threads=229376, block=512, rt_threads=32768,
+                                  frontier_log2=28) as srv:
+                    res = serve.latency_summary(srv.run_trace(rows))
+        out.update(res)
+        for gr in graphs:
+            gr.close()
+    return out
 
 
 def _progress(msg):
@@ -217,9 +320,7 @@ def secondary(g, with_cpu):
             f = q.fifo_check(65536, 32, concurrent=True)
         rows["fifo_check_65536x32_verified"] = bool(f["verified"])
         if with_cpu:
-            # the host share of one GPU is 16 cores whatever os.cpu_count() reports
-            cores = min(len(os.sched_getaffinity(0)), 16)
-            rows["cpu_baseline"] = cpu_baseline(kind, cores, 300, warmup_ms=50)
+            rows["cpu_baseline"] = cpu_baseline(kind, host_cores(), 300, warmup_ms=50)
         sweep[names[kind]] = rows
     out["queue_sweep"] = {"unit": "Mops/s (successful enq + deq)", "chunk_ops": 128, "block": 256,
                           "launches": "5 warmup + 20 timed", "capacity": 1 << 20, "queues": sweep}
@@ -251,13 +352,18 @@ def secondary(g, with_cpu):
             cpu_s = time.perf_counter() - t
             o["cpu_baseline"] = {"value": round(es / cpu_s / 1e6, 2), "unit": "MTEPS", "cores": 1, "kind": "port",
                                  "sample": "one serial textbook BFS (oracle) on the same CSR"}
-            cores = min(len(os.sched_getaffinity(0)), 16)
-            t = time.perf_counter()
-            orc.bfs_parallel(off, idx, 0, threads=cores)
-            par_s = time.perf_counter() - t
+            cores = host_cores()
+            orc.bfs_parallel(off, idx, 0, threads=cores)  # warm: first-touch of the frontier arrays
+            par = []
+            for _ in range(3):
+                t = time.perf_counter()
+                orc.bfs_parallel(off, idx, 0, threads=cores)
+                par.append(time.perf_counter() - t)
+            par_s = float(np.median(par))
             o["cpu_baseline_parallel"] = {"value": round(es / par_s / 1e6, 2), "unit": "MTEPS", "cores": cores,
                                           "kind": "port",
-                                          "sample": "one level-synchronous std::thread BFS (oracle) on the same CSR"}
+                                          "sample": "level-synchronous top-down OpenMP BFS (oracle) on the same CSR, "
+                                                    "OMP_PROC_BIND=close, median of 3 after one warm run"}
     out["bfs_lattice"] = o
     _progress(f"lattice BFS {o['mteps']} MTEPS")
 
