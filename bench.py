@@ -248,7 +248,7 @@ def profile_run(a):
                    frac_p1=(r["enq_success"] + r["deq_success"]) / a.steps / (ms * 1e-3) / p1["tickets_per_s"])
     elif a.profile == "bfs":
         from gcq_amd import bfs
-        mode = {"persistent": bfs.PERSISTENT, "host": bfs.HOST_LOOP}.get(a.mode, a.mode)
+        mode = {"persistent": bfs.PERSISTENT, "host": bfs.HOST_LOOP, "async": bfs.ASYNC}[a.mode]
         with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as gr:
             for _ in range(a.runs):
                 _, res = gr.bfs(0, kind=g.KINDS[a.queue or "glfq"], threads=8192, block=256, mode=mode, depths=False)

@@ -50,6 +50,7 @@ struct Ctl {
   uint32_t err, p3[31];
   uint64_t edges, reached, p4[14];
   int32_t levels, p5[31];
+  uint64_t pending, p6[15];  // barrier-free mode: items pushed and not yet fully expanded
 };
 
 struct LevelAcc {
@@ -238,8 +239,152 @@ template <class F>
 __global__ void k_seed(F q, int32_t* level, int32_t src, Ctl* c) {
   if (threadIdx.x == 0) {
     level[src] = 0;
+    c->pending = 1;
     if (!q.push(0, (uint32_t)src)) c->err = 1;
   }
+}
+
+// ------------------------------------------------------------------ barrier-free mode
+// One frontier queue and no level barrier (GCQ_BFS_ASYNC).  A popped vertex
+// u is expanded with the depth it holds now, d(u) + 1 offered to every
+// neighbour by an unsigned atomic min on its depth word (-1 reads as the
+// largest value, so an unvisited vertex always improves); a neighbour whose
+// depth went down is pushed again.  Depths only fall, and every fall is
+// followed by a push whose expansion reads the lowered value, so at
+// quiescence every edge (u, w) has depth(w) <= depth(u) + 1 and depth(src) =
+// 0: the depths are the BFS depths, bit for bit, whatever the order of
+// work.  Quiescence: `pending` counts items pushed and not yet fully
+// expanded.  A wave adds the items it is about to push before pushing them
+// and gives back the items it popped only with its last push of the batch
+// (or after the batch when it pushes nothing), so pending > 0 while any item
+// is queued or being expanded, and a wave that finds the queue empty may
+// stop once it reads pending == 0.  pending also bounds the items in the
+// ring, which is checked against the ring's capacity before each push.
+constexpr uint32_t kAsyncIdleLimit = 1u << 22;
+
+template <class F>
+__device__ __forceinline__ void async_flush(const F& q, uint32_t tid, uint32_t* stage, uint32_t& sc, uint32_t n,
+                                            uint32_t& owed, Ctl* c, uint64_t cap) {
+  const uint32_t lane = lane_id();
+  uint32_t ok = 1;
+  if (lane == 0) {
+    const uint64_t delta = (uint64_t)n - (uint64_t)owed;  // two's complement: may give items back
+    const uint64_t old = gcq::faa_rlx(&c->pending, delta);
+    if (n && old + delta > cap) {  // the ring could overflow: give the n back and fail the run
+      gcq::faa_rlx(&c->pending, (uint64_t)0 - n);
+      gcq::or_rlx(&c->err, 1u);
+      ok = 0;
+    }
+  }
+  owed = 0;
+  if (!n) return;
+  if (gcq::bcast32(ok, 0)) {
+    flush(q, tid, stage, sc, n, &c->err);
+  } else {
+    const uint32_t rest = sc - n;
+    const uint32_t mv = lane < rest ? stage[64 + lane] : 0u;
+    if (lane < rest) stage[lane] = mv;
+    sc = rest;
+  }
+}
+
+template <class F>
+__global__ void k_async(F q, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx, int32_t* level,
+                        Ctl* c, uint64_t cap) {
+  __shared__ uint32_t stage[16][128];
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = lane_id();
+  uint32_t* const st = stage[(threadIdx.x / 64) & 15];
+  uint32_t* const dep = (uint32_t*)level;
+  uint64_t edges = 0, disc = 0;
+  uint32_t idle = 0;
+  for (;;) {
+    uint32_t u = 0;
+    const bool got = q.pop(tid, &u);
+    const uint64_t gm = __ballot(got);
+    if (gm == 0) {
+      uint32_t stop = 0;
+      if (lane == 0) {
+        stop = gcq::ld_rlx(&c->pending) == 0 || gcq::ld_rlx(&c->lbar.abort) != 0;
+        if (!stop && ++idle > kAsyncIdleLimit) {
+          gcq::st_rlx(&c->lbar.abort, 1u);
+          stop = 1;
+        }
+      }
+      if (gcq::bcast32(stop, 0)) break;
+      const uint32_t nap = gcq::bcast32(idle, 0);
+      for (uint32_t i = 0; i < (nap < 8 ? nap : 8u); ++i) __builtin_amdgcn_s_sleep(2);
+      continue;
+    }
+    idle = 0;
+    uint32_t owed = (uint32_t)__builtin_popcountll(gm);  // popped items, given back with the last push
+    int32_t beg = 0, deg = 0;
+    uint32_t nd = 0;
+    if (got) {
+      beg = offs[u];
+      deg = offs[u + 1] - beg;
+      nd = gcq::ld_rlx(&dep[u]) + 1u;  // the depth now: it may have fallen since the push
+    }
+    edges += (uint64_t)deg;
+    int32_t incl = deg;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t t = __shfl_up(incl, o, 64);
+      if ((int)lane >= o) incl += t;
+    }
+    const int32_t excl = incl - deg;
+    const int32_t total = __shfl(incl, 63, 64);
+    const bool probe = total > kProbeEdges;
+    // this lane's edge target in a round and the depth its owner offers
+    auto neighbour = [&](int32_t base, uint32_t* offer) -> uint32_t {
+      const int32_t e = base + (int32_t)lane;
+      const int32_t ec = e < total ? e : total - 1;
+      int lo = 0, hi = 63;
+      for (int s2 = 0; s2 < 6; ++s2) {
+        const int mid = (lo + hi) >> 1;
+        const int32_t pv = __shfl(incl, mid, 64);
+        if (pv > ec) hi = mid;
+        else lo = mid + 1;
+      }
+      const int32_t obeg = __shfl(beg, lo, 64);
+      const int32_t oexcl = __shfl(excl, lo, 64);
+      *offer = (uint32_t)__shfl((int)nd, lo, 64);
+      return e < total ? (uint32_t)idx[obeg + (e - oexcl)] : 0u;
+    };
+    uint32_t sc = 0;
+    uint32_t off_next = 0;
+    uint32_t w_next = total > 0 ? neighbour(0, &off_next) : 0u;
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t e = base + (int32_t)lane;
+      const uint32_t w = w_next, offer = off_next;
+      if (base + 64 < total) w_next = neighbour(base + 64, &off_next);
+      bool imp = false;
+      if (e < total && (!probe || gcq::ld_cached(&dep[w]) > offer)) {
+        const uint32_t old = __hip_atomic_fetch_min(&dep[w], offer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        imp = old > offer;
+        disc += old == 0xFFFFFFFFu ? 1u : 0u;
+      }
+      const uint64_t dm = __ballot(imp);
+      if (imp) st[sc + gcq::rank_in(dm)] = w;
+      sc += (uint32_t)__builtin_popcountll(dm);
+      if (sc >= 64) async_flush(q, tid, st, sc, 64, owed, c, cap);
+    }
+    async_flush(q, tid, st, sc, sc, owed, c, cap);
+  }
+  edges = gcq::wave_sum(edges);
+  disc = gcq::wave_sum(disc);
+  if (lane == 0) {
+    if (edges) gcq::faa_rlx(&c->edges, edges);
+    if (disc) gcq::faa_rlx(&c->reached, disc);
+  }
+}
+
+// Levels of a finished barrier-free run: the largest depth + 1.
+__global__ void k_levels(const int32_t* level, uint64_t n, Ctl* c) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, stp = (uint64_t)gridDim.x * blockDim.x;
+  int32_t m = -1;
+  for (uint64_t i = gid; i < n; i += stp) m = max(m, level[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if (lane_id() == 0 && m >= 0) __hip_atomic_fetch_max(&c->levels, m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace bfsk
@@ -295,7 +440,20 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
   GCQ_HIP(hipEventCreate(&e0));
   GCQ_HIP(hipEventCreate(&e1));
   int rc = GCQ_OK;
-  if (cfg->mode == GCQ_BFS_PERSISTENT) {
+  if (cfg->mode == GCQ_BFS_ASYNC) {
+    if constexpr (F::kQueues == 3) {
+      gcqh::set_error("the barrier-free BFS needs a ring frontier (G-WFQ, G-LFQ or SFQ); WFQ cells are not reused");
+      rc = GCQ_EINVAL;
+    } else {
+      const uint32_t nv = (uint32_t)g->host.nv;
+      GCQ_HIP(hipEventRecord(e0, g->stream));
+      hipLaunchKernelGGL(bfsk::k_async<F>, dim3(grid), dim3(cfg->block), 0, g->stream, fq.q[0], g->d_off, g->d_idx,
+                         c.level, ctl, (uint64_t)c.fs.capacity);
+      GCQ_HIP(hipEventRecord(e1, g->stream));
+      hipLaunchKernelGGL(bfsk::k_levels, dim3(gcqf::grid_of(nv)), dim3(256), 0, g->stream, c.level, (uint64_t)nv,
+                         ctl);
+    }
+  } else if (cfg->mode == GCQ_BFS_PERSISTENT) {
     int per_cu = 0, dev = 0, cus = 0;
     GCQ_HIP(hipGetDevice(&dev));
     GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -448,7 +606,8 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   GCQ_REQUIRE(cfg->block >= 64 && cfg->block % 64 == 0 && cfg->block <= 1024, "block must be a multiple of 64");
   GCQ_REQUIRE(cfg->threads >= cfg->block && cfg->threads % cfg->block == 0,
               "threads must be a positive multiple of block (whole waves, bfs_bench.cpp:521)");
-  GCQ_REQUIRE(cfg->mode == GCQ_BFS_HOST_LOOP || cfg->mode == GCQ_BFS_PERSISTENT, "bad mode");
+  GCQ_REQUIRE(cfg->mode == GCQ_BFS_HOST_LOOP || cfg->mode == GCQ_BFS_PERSISTENT || cfg->mode == GCQ_BFS_ASYNC,
+              "bad mode");
   std::memset(res, 0, sizeof *res);
   if (src < 0 || src >= g->host.nv) src = 0;  // bfs_bench.cpp:555
   int rc = gcqg::ensure_uploaded(g);

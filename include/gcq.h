@@ -214,7 +214,11 @@ int gcq_graph_destroy(gcq_graph* g);
 /* ------------------------------------------------------------------ BFS */
 enum gcq_bfs_mode {
   GCQ_BFS_HOST_LOOP = 0,  /* one launch + host sync per level (bfs_bench.cpp:583-623) */
-  GCQ_BFS_PERSISTENT = 1  /* one launch, device-side level barrier */
+  GCQ_BFS_PERSISTENT = 1, /* one launch, device-side level barrier */
+  GCQ_BFS_ASYNC = 2       /* one launch, no level barrier: one frontier queue, a vertex whose depth falls
+                             (unsigned atomic min) is pushed again, the run ends when no pushed item is left
+                             unexpanded.  Same depths bit for bit; edges_scanned counts every expansion,
+                             repeated ones included.  Ring frontiers only (G-WFQ, G-LFQ, SFQ). */
 };
 typedef struct gcq_bfs_config {
   int32_t kind;       /* frontier queue: enum gcq_kind */

@@ -8,7 +8,8 @@ pytestmark = pytest.mark.gpu
 
 KINDS = [0, 1, 2, 3]
 NAMES = {0: "gwfq", 1: "glfq", 2: "wfq", 3: "sfq"}
-MODES = {0: "hostloop", 1: "persistent"}
+MODES = {0: "hostloop", 1: "persistent", 2: "async"}
+RING_KINDS = [0, 1, 3]  # the barrier-free mode takes ring frontiers
 
 
 def _csr(nv, u, v):
@@ -30,14 +31,19 @@ def _run_and_check(orc, g, src, kind, threads, mode, off=None, idx=None):
     bad = np.flatnonzero(d != ref)
     assert len(bad) == 0, f"{len(bad)} depths differ, first at {bad[:5]}: got {d[bad[:5]]} want {ref[bad[:5]]}"
     assert res["reached"] == reached
-    assert res["edges_scanned"] == es
+    if mode == 2:  # every expansion counts, a vertex expanded again after its depth fell included
+        assert res["edges_scanned"] >= es
+    else:
+        assert res["edges_scanned"] == es
     assert res["levels"] == int(ref.max()) + 1
     return res
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=[MODES[m] for m in (0, 1)])
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=[MODES[m] for m in (0, 1, 2)])
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
 def test_lattice_small(gcq, orc, kind, mode):
+    if mode == 2 and kind == 2:
+        pytest.skip("the barrier-free mode takes ring frontiers (WFQ is rejected: test_async_rejects_wfq)")
     from gcq_amd import bfs
     with bfs.Graph.lattice(64, 64, 0.75, 0.01, 7) as g:
         off, idx = g.csr()
@@ -63,7 +69,7 @@ def test_special_graphs(gcq, orc, kind):
     for name, ((off, idx), src) in cases.items():
         print(f"[bfs special] {name}", flush=True)
         with bfs.Graph.from_csr(off, idx) as g:
-            for mode in (0, 1):
+            for mode in ((0, 1, 2) if kind in RING_KINDS else (0, 1)):
                 _run_and_check(orc, g, src, kind, 1024, mode, off, idx)
 
 
@@ -77,6 +83,8 @@ def test_rmat_hubs(gcq, orc, kind):
         for k in range(2):
             _run_and_check(orc, g, g.pick_source(11, k), kind, 8192, 1, off, idx)
         _run_and_check(orc, g, g.pick_source(11, 2), kind, 8192, 0, off, idx)
+        if kind in RING_KINDS:
+            _run_and_check(orc, g, g.pick_source(11, 3), kind, 8192, 2, off, idx)
 
 
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
@@ -101,13 +109,23 @@ def test_wfq_cells_recycled_65536_threads(gcq, orc, mode):
 @pytest.mark.parametrize("kind", [1, 0, 2, 3], ids=["glfq", "gwfq", "wfq", "sfq"])
 def test_config3_lattice_4900(gcq, orc, kind):
     # The benchmark graph: 4900 x 4900 lattice, p = 0.75, 1% diagonals,
-    # seed 7, source 0, 8,192 threads (block 256), persistent: 9,371 levels
-    # of counted pops, each level's Head set back at its barrier.
+    # seed 7, source 0, 8,192 threads (block 256): persistent (9,371 levels
+    # of counted pops, each level's Head set back at its barrier) and, for the
+    # ring frontiers, barrier-free.
     from gcq_amd import bfs
     with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as g:
-        res = _run_and_check(orc, g, 0, kind, 8192, 1)
-        print(f"[bfs cfg3] levels={res['levels']} reached={res['reached']} kernel_ms={res['kernel_ms']:.1f} "
-              f"MTEPS={res['edges_scanned'] / res['kernel_ms'] / 1e3:.1f}", flush=True)
+        off, idx = g.csr()
+        for mode in ((1, 2) if kind in RING_KINDS else (1,)):
+            res = _run_and_check(orc, g, 0, kind, 8192, mode, off, idx)
+            print(f"[bfs cfg3] mode={MODES[mode]} levels={res['levels']} reached={res['reached']} "
+                  f"kernel_ms={res['kernel_ms']:.1f} scanned={res['edges_scanned']}", flush=True)
+
+
+def test_async_rejects_wfq(gcq):
+    from gcq_amd import GcqError, bfs
+    with bfs.Graph.lattice(16, 16, 0.75, 0.01, 7) as g:
+        with pytest.raises(GcqError, match="ring frontier"):
+            g.bfs(0, kind=2, threads=256, mode=2)
 
 
 def test_kind_switch_same_graph(gcq, orc):
