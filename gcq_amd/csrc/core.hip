@@ -540,7 +540,7 @@ extern "C" int gcq_tb_run(gcq_queue* q, const gcq_tb_config* cfg, gcq_tb_result*
         res->launches = launches;
         return GCQ_OK;
       },
-      /*exact_empty=*/false);
+      /*exact_empty=*/false, /*blocking=*/cfg->mode == GCQ_TB_BALANCED);
   if (rc == GCQ_OK) {
     std::vector<uint64_t> h(4ull * grid);
     if (hipMemcpy(h.data(), stats, 32ull * grid, hipMemcpyDeviceToHost) != hipSuccess) rc = GCQ_EHIP;

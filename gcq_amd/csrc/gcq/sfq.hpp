@@ -14,7 +14,8 @@
 //    transfer touch one line.
 //  * blocking tickets come from one aggregated FAA per wave; non-blocking
 //    claims are wave-batched: the active lanes test consecutive tickets
-//    t0+rank and one CAS per wave claims the ready prefix [t0, t0+k).
+//    t0+rank and one CAS per wave claims the ready prefix [t0, t0+k), or
+//    one CAS per workgroup when the caller passes a Combine (nb_claim_group).
 //  * Tail, Head and the closed flag sit on separate 128-B lines.
 #pragma once
 #include "prim.hpp"
@@ -87,6 +88,14 @@ __device__ __forceinline__ void close(Queue* q) { st_rlx(closed(q), 1u); }
 __device__ __forceinline__ void backoff(uint32_t iters) {
   for (uint32_t i = 0; i < iters; ++i) __builtin_amdgcn_s_sleep(1);
 }
+// Wait between checks of a slot's phase in the blocking operations: short
+// for the first kQuickChecks checks (the slot's other party is usually
+// already past its ticket), then the queue's polite backoff.
+constexpr uint64_t kQuickChecks = 64;
+__device__ __forceinline__ void slot_wait(uint64_t spins, uint32_t iters) {
+  if (spins < kQuickChecks) __builtin_amdgcn_s_sleep(1);
+  else backoff(iters);
+}
 
 // Blocking enqueue (sfqueue_hip.cpp:55-70).  `spin_limit` bounds the wait
 // (0 = unbounded); exceeding it returns kBusy so no caller hangs forever.
@@ -102,7 +111,7 @@ __device__ inline int enqueue_blocking(Queue* q, uint64_t item, uint64_t spin_li
   while (ld_rlx(&s->id) != id) {  // the previous lap's dequeuer has released the slot
     if (is_closed(q)) return kClosed;
     if (spin_limit && ++spins > spin_limit) return kBusy;
-    backoff(g.backoff);
+    slot_wait(spins, g.backoff);
   }
   st_rlx(&s->item, item);
   vm_wait();  // item acknowledged before the phase flip publishes it
@@ -122,7 +131,7 @@ __device__ inline int dequeue_blocking(Queue* q, uint64_t* out, uint64_t spin_li
   while (ld_rlx(&s->id) != id) {
     if (is_closed(q)) return kClosed;
     if (spin_limit && ++spins > spin_limit) return kBusy;
-    backoff(g.backoff);
+    slot_wait(spins, g.backoff);
   }
   *out = ld_rlx(&s->item);
   vm_wait();  // item read completes before the slot is handed to the next lap
@@ -180,11 +189,115 @@ __device__ __forceinline__ uint64_t nb_claim(Queue* q, uint64_t* ctr, uint64_t p
   return (won && rank_in(m) < k) ? t : ~0ull;
 }
 
-// Non-blocking enqueue (sfqueue_hip.cpp:89-107).
+// The non-blocking claim shared by the waves of a workgroup that claim at
+// about the same time (the same LDS group protocol as combine_claim in
+// prim.hpp: join, the leader closes the group after the previous group's
+// claim, members wait for the leader's result).  The leading wave reads the
+// counter once, tests the phase of the group's N tickets t0 .. t0 + N - 1
+// with all its active lanes (up to 4 loads per lane, issued together), and
+// claims the ready prefix [t0, t0 + k) with one CAS; lane r of a member at
+// offset o gets t0 + o + r when o + r < k.  One CAS per workgroup instead of
+// one per wave thins the contention on the counter line, and a won CAS hands
+// out up to a whole workgroup's tickets.  Every member's operation is in
+// progress between the read and the CAS, as in the per-wave claim.
+constexpr uint32_t kGroupProbeRounds = 4;
+__device__ inline uint64_t nb_claim_group(Queue* q, uint64_t* ctr, uint64_t phase_add, Combine* c) {
+  const Geometry g = q->g;
+  const uint64_t m = active_mask();
+  const uint32_t leader = first_lane(m);
+  const uint32_t n = (uint32_t)__builtin_popcountll(m);
+  const uint32_t rank = rank_in(m);
+  const uint32_t w = wave_in_block();
+  uint32_t grp = 0, off = 0;
+  if (lane_id() == leader) {
+    __hip_atomic_store(&c->res_g[w], ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint64_t old = __hip_atomic_fetch_add(&c->join, (1ull << (32 + w)) + n, __ATOMIC_RELEASE,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+    grp = (uint32_t)(old >> 48);
+    off = (uint32_t)old;
+  }
+  grp = bcast32(grp, leader);
+  off = bcast32(off, leader);
+  uint32_t k = 0;
+  uint64_t base = 0;
+  if (off == 0) {
+    uint64_t fin = 0;
+    if (lane_id() == leader) {
+      for (uint32_t sp = 0; sp < kCombineLeaderSpins &&
+                            (__hip_atomic_load(&c->done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xFFFFu) !=
+                                ((grp - 1u) & 0xFFFFu);
+           ++sp)
+        __builtin_amdgcn_s_sleep(1);
+      fin = __hip_atomic_exchange(&c->join, (uint64_t)((grp + 1u) & 0xFFFFu) << 48, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+      lgkm_wait();  // the group is closed before the counter is read
+      base = ld_rlx(ctr);
+    }
+    fin = bcast64(fin, leader);
+    base = bcast64(base, leader);
+    const uint32_t total = (uint32_t)fin;
+    const uint32_t cover = total < kGroupProbeRounds * n ? total : kGroupProbeRounds * n;
+    bool ready[kGroupProbeRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kGroupProbeRounds; ++r) {
+      const uint32_t j = r * n + rank;
+      const uint64_t t = base + j;
+      ready[r] = j >= cover || ld_rlx(&slots(q)[t & (g.qlen - 1)].id) == 2ull * (t >> g.qlen_log2) + phase_add;
+    }
+    k = cover;
+#pragma unroll
+    for (uint32_t r = 0; r < kGroupProbeRounds; ++r) {
+      const uint64_t bad = m & ~__ballot(ready[r]);
+      if (bad && k == cover) {  // the smallest unready ticket of the round: the lowest unready lane's rank
+        const uint32_t fl = first_lane(bad);
+        k = r * n + (uint32_t)__builtin_popcountll(m & ((1ull << fl) - 1ull));
+      }
+    }
+    uint32_t won = 0;
+    if (lane_id() == leader && k > 0) {
+      uint64_t exp = base;
+      won = cas_rlx(ctr, exp, base + k) ? 1u : 0u;
+    }
+    won = bcast32(won, leader);
+    if (k > 0) {  // the per-wave contention backoff, for the leading wave
+      uint8_t* const bo = busy_backoff();
+      const uint32_t e0 = *bo & 15u, e = e0 < kBusyBackoffMax ? e0 : kBusyBackoffMax;
+      if (!won) {
+        for (uint32_t i = 0; i < (1u << e); ++i) __builtin_amdgcn_s_sleep(1);
+        if (lane_id() == leader) *bo = (uint8_t)(e < kBusyBackoffMax ? e + 1u : e);
+      } else if (lane_id() == leader) {
+        *bo = (uint8_t)(e ? e - 1u : 0u);
+      }
+    }
+    if (!won) k = 0;
+    if (lane_id() == leader) {
+      for (uint32_t b = (uint32_t)(fin >> 32) & 0xFFFFu & ~(1u << w); b; b &= b - 1) {
+        const uint32_t x = (uint32_t)__builtin_ctz(b);
+        c->res_k[x] = k;
+        c->res_base[x] = base;
+        __hip_atomic_store(&c->res_g[x], grp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      __hip_atomic_store(&c->done, grp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  } else {
+    if (lane_id() == leader) {
+      while (__hip_atomic_load(&c->res_g[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != grp)
+        __builtin_amdgcn_s_sleep(1);
+      k = c->res_k[w];
+      base = c->res_base[w];
+    }
+    k = bcast32(k, leader);
+    base = bcast64(base, leader);
+  }
+  return off + rank < k ? base + off + rank : ~0ull;
+}
+
+// Non-blocking enqueue (sfqueue_hip.cpp:89-107).  With a Combine the claim
+// is shared by the workgroup's enqueuing waves (nb_claim_group).
 template <class Probe = NoProbe>
-__device__ inline int enqueue_nb(Queue* q, uint64_t item, Probe* pr = nullptr) {
+__device__ inline int enqueue_nb(Queue* q, uint64_t item, Probe* pr = nullptr, Combine* cg = nullptr) {
   if (is_closed(q)) return kClosed;
-  const uint64_t t = nb_claim(q, tail(q), 0);
+  const uint64_t t = cg ? nb_claim_group(q, tail(q), 0, cg) : nb_claim(q, tail(q), 0);
   if (pr) pr->step();
   if (t == ~0ull) return kBusy;
   Slot* s = &slots(q)[t & (q->g.qlen - 1)];
@@ -194,11 +307,12 @@ __device__ inline int enqueue_nb(Queue* q, uint64_t item, Probe* pr = nullptr) {
   return kSuccess;
 }
 
-// Non-blocking dequeue (sfqueue_hip.cpp:109-128).
+// Non-blocking dequeue (sfqueue_hip.cpp:109-128).  With a Combine the claim
+// is shared by the workgroup's dequeuing waves.
 template <class Probe = NoProbe>
-__device__ inline int dequeue_nb(Queue* q, uint64_t* out, Probe* pr = nullptr) {
+__device__ inline int dequeue_nb(Queue* q, uint64_t* out, Probe* pr = nullptr, Combine* cg = nullptr) {
   if (is_closed(q)) return kClosed;
-  const uint64_t t = nb_claim(q, head(q), 1);
+  const uint64_t t = cg ? nb_claim_group(q, head(q), 1, cg) : nb_claim(q, head(q), 1);
   if (pr) pr->step();
   if (t == ~0ull) return kBusy;
   Slot* s = &slots(q)[t & (q->g.qlen - 1)];
@@ -213,9 +327,10 @@ __device__ inline int dequeue_nb(Queue* q, uint64_t* out, Probe* pr = nullptr) {
 // BFS adapter rule of the reference (bfs_bench.cpp:114-131) and gives
 // dequeue results the FIFO model can check.
 template <class Probe = NoProbe>
-__device__ inline bool dequeue_or_empty(Queue* q, uint64_t* out, uint64_t retry_limit, Probe* pr = nullptr) {
+__device__ inline bool dequeue_or_empty(Queue* q, uint64_t* out, uint64_t retry_limit, Probe* pr = nullptr,
+                                        Combine* cg = nullptr) {
   for (uint64_t k = 0;; ++k) {
-    const int rc = dequeue_nb(q, out, pr);
+    const int rc = dequeue_nb(q, out, pr, cg);
     if (rc == kSuccess) return true;
     if (rc == kClosed) return false;
     const uint64_t h = ld_rlx(head(q));

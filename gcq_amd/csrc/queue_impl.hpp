@@ -63,25 +63,37 @@ struct LfqQ {
   }
 };
 
-// SFQ: enqueue is one non-blocking attempt (BUSY = legal failed enqueue).
-// Dequeue is one non-blocking attempt in throughput mode (the reference's
-// split harness counts BUSY as empty, throughout_bench.cpp:275-285); with
-// exact_empty set it retries BUSY until it succeeds or observes
-// head >= tail, so an EMPTY result means the queue was empty.
+// SFQ, as the reference's harness adapters use it
+// (throughout_bench.cpp:251-285):
+//  - blocking = 1 (the balanced harness): sfq_enqueue_blocking /
+//    sfq_dequeue_blocking, each waiting for its ticket's slot; the wait is
+//    bounded (kBlockingSpins checks) and answers BUSY past it, where the
+//    reference waits forever;
+//  - otherwise enqueue is one non-blocking attempt (BUSY = a legal failed
+//    enqueue) and dequeue one non-blocking attempt (the split harness counts
+//    BUSY as empty); with exact_empty set the dequeue retries BUSY until it
+//    succeeds or observes head >= tail, so an EMPTY result means the queue
+//    was empty.
+// A Combine passed to a non-blocking operation shares its claim with the
+// workgroup's other waves (nb_claim_group).
 struct SfqQ {
+  static constexpr uint64_t kBlockingSpins = 1ull << 16;
   gcq::sfq::Queue* q;
   uint32_t exact_empty;
+  uint32_t blocking;
   template <class P = gcq::sfq::NoProbe>
-  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr, gcq::Combine* = nullptr) const {
-    return gcq::sfq::enqueue_nb(q, v, p) == gcq::sfq::kSuccess;
+  __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
+    if (blocking) return gcq::sfq::enqueue_blocking(q, v, kBlockingSpins, p) == gcq::sfq::kSuccess;
+    return gcq::sfq::enqueue_nb(q, v, p, ct) == gcq::sfq::kSuccess;
   }
   __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, decltype(nullptr), gcq::Combine* ct) const {
     return enq(tid, v, (gcq::sfq::NoProbe*)nullptr, ct);
   }
   template <class P = gcq::sfq::NoProbe>
-  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr, gcq::Combine* = nullptr) const {
-    if (exact_empty) return gcq::sfq::dequeue_or_empty(q, o, 1u << 20, p);
-    return gcq::sfq::dequeue_nb(q, o, p) == gcq::sfq::kSuccess;
+  __device__ __forceinline__ bool deq(uint32_t, uint64_t* o, P* p = nullptr, gcq::Combine* cb = nullptr) const {
+    if (blocking) return gcq::sfq::dequeue_blocking(q, o, kBlockingSpins, p) == gcq::sfq::kSuccess;
+    if (exact_empty) return gcq::sfq::dequeue_or_empty(q, o, 1u << 20, p, cb);
+    return gcq::sfq::dequeue_nb(q, o, p, cb) == gcq::sfq::kSuccess;
   }
   __device__ __forceinline__ bool deq(uint32_t tid, uint64_t* o, decltype(nullptr), gcq::Combine* cb) const {
     return deq(tid, o, (gcq::sfq::NoProbe*)nullptr, cb);
@@ -114,8 +126,9 @@ struct YmcQ {
 
 // Call f(adapter) with the adapter matching the queue kind.
 // exact_empty selects SFQ's checked-empty dequeue (correctness harnesses).
+// blocking selects SFQ's blocking operations (the balanced throughput harness).
 template <class F>
-int gcq_dispatch(gcq_queue* q, F&& f, bool exact_empty = true) {
+int gcq_dispatch(gcq_queue* q, F&& f, bool exact_empty = true, bool blocking = false) {
   switch (q->cfg.kind) {
     case GCQ_GWFQ: {
       gcqk::GwfQ a{gcq::gwf::mpmc_view(q->dev, q->cfg.capacity, q->cfg.max_threads)};
@@ -126,7 +139,7 @@ int gcq_dispatch(gcq_queue* q, F&& f, bool exact_empty = true) {
       return f(a);
     }
     case GCQ_SFQ: {
-      gcqk::SfqQ a{(gcq::sfq::Queue*)q->dev, exact_empty ? 1u : 0u};
+      gcqk::SfqQ a{(gcq::sfq::Queue*)q->dev, exact_empty ? 1u : 0u, blocking ? 1u : 0u};
       return f(a);
     }
     case GCQ_WFQ: {

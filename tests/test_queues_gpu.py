@@ -119,7 +119,7 @@ def test_grouped_tail_tickets_conserve_items(gcq, kind, mode):
         _conserved(q, r, f"tb grouped {NAMES[kind]}")
 
 
-@pytest.mark.parametrize("kind", [0, 1], ids=["gwfq", "glfq"])
+@pytest.mark.parametrize("kind", [0, 1, 3], ids=["gwfq", "glfq", "sfq"])
 @pytest.mark.parametrize("pp", [25, 75])
 def test_combined_claims_split_histories(gcq, orc, kind, pp):
     # 8,192 threads in blocks of 256 (4 waves), split roles: idle consumer
