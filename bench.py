@@ -307,9 +307,8 @@ def secondary(g, with_cpu):
     names = {g.GWFQ: "gwfq", g.GLFQ: "glfq", g.WFQ: "wfq", g.SFQ: "sfq"}
     for kind in (g.GWFQ, g.GLFQ, g.WFQ, g.SFQ):
         rows = {}
-        # WFQ cells are not recycled in this harness: 2^29 cells (8 GiB) cover
-        # 25 balanced launches at 65,536 threads; 2^28 ran out in the last one.
-        extra = {"wfq_segment_cells": 1 << 29} if kind == g.WFQ else {}
+        # WFQ's cell pool is its default: 2^20 cells, a ring reclaimed as it runs
+        extra = {}
         for mode, mname in ((g.SPLIT, "split25"), (0, "balanced")):
             for T in (1024, 2048, 4096, 8192, 16384, 32768, 65536):
                 with g.Queue(kind, capacity=1 << 20, max_threads=65536, poll_backoff=0, **extra) as q:

@@ -85,10 +85,8 @@ __global__ void k_zero_u64(uint64_t* p, uint64_t words) {
 __global__ void k_init_ymc(void* base, void* handles, uint64_t cells, uint32_t nthreads, uint32_t patience) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
   gcq::ymc::init_slice((gcq::ymc::Queue*)base, (gcq::ymc::Handle*)handles, cells, nthreads, patience, gid, st);
-}
-
-__global__ void k_ymc_high_water(void* base, uint64_t* out) {
-  *out = gcq::ymc::high_water((gcq::ymc::Queue*)base);
+  // library queues reclaim their cells as they run (ring mode)
+  gcq::ymc::init_ring_slice((gcq::ymc::Queue*)base, (gcq::ymc::Handle*)handles, cells, nthreads, gid, st);
 }
 
 bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
@@ -118,18 +116,8 @@ int launch_init(gcq_queue* q) {
                          (uint64_t)c.capacity, c.max_threads);
       break;
     case GCQ_WFQ: {
-      // Zero the touched prefix of the cell pool (all of it on first use).
-      uint64_t upto = c.wfq_segment_cells;
-      if (q->aux_bytes) {  // aux_bytes != 0 marks a queue that ran before
-        uint64_t* d_hw = nullptr;
-        GCQ_HIP(hipMallocAsync((void**)&d_hw, 8, q->stream));
-        hipLaunchKernelGGL(k_ymc_high_water, dim3(1), dim3(1), 0, q->stream, q->dev, d_hw);
-        uint64_t hw = 0;
-        GCQ_HIP(hipMemcpyAsync(&hw, d_hw, 8, hipMemcpyDeviceToHost, q->stream));
-        GCQ_HIP(hipStreamSynchronize(q->stream));
-        GCQ_HIP(hipFreeAsync(d_hw, q->stream));
-        upto = std::min<uint64_t>(hw + 1024, c.wfq_segment_cells);
-      }
+      // Zero the whole cell pool: it is a ring, every cell may have been used.
+      const uint64_t upto = c.wfq_segment_cells;
       uint64_t* cells = (uint64_t*)((char*)q->dev + gcq::ymc::kHdrBytes);
       hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(upto * 2, 256, 65536)), dim3(256), 0, q->stream, cells, upto * 2);
       hipLaunchKernelGGL(k_init_ymc, dim3(grid_for(c.max_threads)), dim3(256), 0, q->stream, q->dev, q->aux,
@@ -182,7 +170,20 @@ extern "C" int gcq_queue_create(const gcq_queue_config* cfg, gcq_queue** out) {
       GCQ_REQUIRE(is_pow2(c.capacity) && c.capacity >= 64, "SFQ capacity must be a power of two >= 64");
       break;
     case GCQ_WFQ:
-      if (c.wfq_segment_cells == 0) c.wfq_segment_cells = 1u << 26;
+      // The cell pool is a ring of segments reclaimed as the queue runs.  It
+      // must hold the queue's items plus the reclamation lag; by default it
+      // holds max(capacity, 32 cells per thread) cells, a power of two: 2^21
+      // (32 MiB) at 65,536 threads, where the split harness refuses no
+      // enqueue (at 2^20 cells it refused 1-2% as full in some launches).
+      if (c.wfq_segment_cells == 0) {
+        const uint64_t want = std::max<uint64_t>({(uint64_t)c.capacity, 64ull * c.max_threads, 1ull << 12});
+        uint64_t p2 = 1;
+        while (p2 < want) p2 <<= 1;
+        GCQ_REQUIRE(p2 <= (1ull << 31), "WFQ cell pool too large");
+        c.wfq_segment_cells = (uint32_t)p2;
+      }
+      GCQ_REQUIRE(is_pow2(c.wfq_segment_cells) && c.wfq_segment_cells >= (1u << 12),
+                  "WFQ cell pool must be a power of two >= 4096 cells");
       GCQ_REQUIRE(c.enq_patience <= 1024, "WFQ patience too large");
       break;
     default:
@@ -217,14 +218,15 @@ extern "C" int gcq_queue_create(const gcq_queue_config* cfg, gcq_queue** out) {
     return fail(GCQ_ENOMEM);
   }
   if (c.kind == GCQ_WFQ) {
-    if (hipMalloc(&q->aux, sizeof(gcq::ymc::Handle) * (size_t)c.max_threads) != hipSuccess) {
+    // handles, then one reclamation floor per wave
+    q->aux_bytes = sizeof(gcq::ymc::Handle) * (uint64_t)c.max_threads + 8ull * gcq::ymc::wave_slots(c.max_threads);
+    if (hipMalloc(&q->aux, q->aux_bytes) != hipSuccess) {
       gcqh::set_error("hipMalloc for WFQ handles failed");
       return fail(GCQ_ENOMEM);
     }
   }
   int rc = launch_init(q);
   if (rc) return fail(rc);
-  if (c.kind == GCQ_WFQ) q->aux_bytes = sizeof(gcq::ymc::Handle) * (uint64_t)c.max_threads;
   *out = q;
   return GCQ_OK;
 }
@@ -408,6 +410,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
   gcq::Combine* const cb = solo || gridDim.x * blockDim.x < gcq::kCombineMinThreads ? nullptr : &s_cb;
   gcq::Combine* const ct = kGroupEnq ? &s_ct : nullptr;
   uint64_t eo = 0, ef = 0, dok = 0, de = 0;
+  q.begin(tid);
   if (tid < threads) {
     const uint64_t seq0 = (uint64_t)launch * chunk + 1;
     if (mode == GCQ_TB_SPLIT) {
@@ -430,6 +433,7 @@ __global__ void k_tb(Q q, uint32_t threads, uint32_t chunk, int32_t mode, uint32
       }
     }
   }
+  q.end(tid);
   __shared__ uint64_t red[4][16];
   const uint32_t w = threadIdx.x / gcq::kWave;
   eo = gcq::wave_sum(eo);
@@ -568,10 +572,12 @@ template <class Q>
 __global__ void k_drain(Q q, uint32_t threads, uint64_t* count) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t n = 0;
+  q.begin(tid);
   if (tid < threads) {
     uint64_t v;
     while (q.deq(tid, &v)) ++n;
   }
+  q.end(tid);
   n = gcq::wave_sum(n);
   if (gcq::lane_id() == 0 && n) gcq::faa_rlx(count, n);
 }
@@ -652,29 +658,34 @@ __device__ void fifo_take(const Q& q, uint32_t tid, const FifoDev& f, uint32_t p
 template <class Q>
 __global__ void k_fifo_produce(Q q, uint32_t producers, uint32_t ops, uint32_t* fail) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= producers) return;
-  for (uint32_t i = 0; i < ops; ++i) {
-    const uint64_t tok = ((uint64_t)tid << 32) | (uint64_t)(i + 1u);
-    uint32_t spins = 0;
-    while (!q.enq(tid, tok)) {
-      if (++spins > kFifoSpinLimit) {
-        gcq::faa_rlx(fail, 1u);
-        return;
+  q.begin(tid);
+  if (tid < producers) {
+    for (uint32_t i = 0; i < ops; ++i) {
+      const uint64_t tok = ((uint64_t)tid << 32) | (uint64_t)(i + 1u);
+      uint32_t spins = 0;
+      while (!q.enq(tid, tok)) {
+        if (++spins > kFifoSpinLimit) {
+          gcq::faa_rlx(fail, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
       }
-      __builtin_amdgcn_s_sleep(2);
+      if (spins > kFifoSpinLimit) break;
     }
   }
+  q.end(tid);
 }
 
 template <class Q>
 __global__ void k_fifo_drain(Q q, uint32_t threads, FifoDev f, uint32_t producers, uint32_t ops, uint64_t total) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= threads) return;
+  q.begin(tid);
   uint32_t streak = 0;
-  while (gcq::ld_rlx(f.consumed) < total && streak < kFifoSpinLimit) {
+  while (tid < threads && gcq::ld_rlx(f.consumed) < total && streak < kFifoSpinLimit) {
     fifo_take(q, tid, f, producers, ops, total, &streak);
     if (streak) __builtin_amdgcn_s_sleep(2);
   }
+  q.end(tid);
 }
 
 // Producers (tid*100/threads < 25) and consumers in one launch.
@@ -689,6 +700,7 @@ __global__ void k_fifo_mixed(Q q, uint32_t threads, uint32_t producers, FifoDev 
   }
   __syncthreads();
   if (tid >= threads) return;
+  q.begin(tid);
   if (tid < producers) {
     for (uint32_t i = 0; i < ops; ++i) {
       const uint64_t tok = ((uint64_t)tid << 32) | (uint64_t)(i + 1u);
@@ -696,11 +708,13 @@ __global__ void k_fifo_mixed(Q q, uint32_t threads, uint32_t producers, FifoDev 
       while (!q.enq(tid, tok, nullptr, &s_ct)) {
         if (++spins > kFifoSpinLimit) {
           gcq::faa_rlx(fail, 1u);
-          return;
+          break;
         }
         __builtin_amdgcn_s_sleep(2);
       }
+      if (spins > kFifoSpinLimit) break;
     }
+    q.end(tid);
     return;
   }
   uint32_t streak = 0;
@@ -708,6 +722,7 @@ __global__ void k_fifo_mixed(Q q, uint32_t threads, uint32_t producers, FifoDev 
     fifo_take(q, tid, f, producers, ops, total, &streak, &s_cb);
     if (streak) __builtin_amdgcn_s_sleep(2);
   }
+  q.end(tid);
 }
 
 __global__ void k_fifo_verify(const uint32_t* counts, uint64_t n, uint32_t* zeros, uint32_t* gt1) {
@@ -751,9 +766,9 @@ extern "C" int gcq_fifo_check(gcq_queue* q, uint32_t threads, uint32_t ops, uint
   const uint32_t producers = concurrent ? (uint32_t)(((uint64_t)threads * 25 + 99) / 100) : threads;
   GCQ_REQUIRE(producers >= 1, "no producers");
   const uint64_t total = (uint64_t)producers * ops;
-  if (!concurrent && q->cfg.kind != GCQ_WFQ)
-    GCQ_REQUIRE(total <= q->cfg.capacity, "produce-then-drain needs threads*ops <= capacity (%llu > %u)",
-                (unsigned long long)total, q->cfg.capacity);
+  if (!concurrent)
+    GCQ_REQUIRE(total <= (q->cfg.kind == GCQ_WFQ ? q->cfg.wfq_segment_cells : q->cfg.capacity),
+                "produce-then-drain needs threads*ops <= capacity (%llu items)", (unsigned long long)total);
   res->in_count = total;
   const uint64_t bytes = 16ull * total + 4ull * total + 256;
   char* buf = nullptr;
@@ -838,6 +853,7 @@ __global__ void k_hist(Q q, gcq_hist_config c, uint32_t stride, LogEntry* log, u
   }
   __syncthreads();
   if (tid >= c.threads) return;
+  q.begin(tid);
   const bool victim = c.delay_every && ((tid + (uint32_t)c.seed) % c.delay_every == 0);
   const bool prod = ((uint64_t)tid * 100u) / c.threads < c.producer_percent;
   LogEntry* my = log + (uint64_t)tid * stride;
@@ -878,6 +894,7 @@ __global__ void k_hist(Q q, gcq_hist_config c, uint32_t stride, LogEntry* log, u
       my[k++] = e;
     }
   }
+  q.end(tid);
 }
 
 }  // namespace

@@ -17,6 +17,29 @@
 //    exhausted pool makes enqueue return false and dequeue EMPTY, as in the
 //    reference, and the host resets only the touched prefix.
 //  * T and H tickets of the fast paths come from one aggregated FAA per wave.
+//  * Ring mode (library queues): the pool is a ring of P segments of S cells
+//    and cell id i lives at i mod (P * S), so a bounded pool serves an
+//    unbounded run.  The reference declares hazard pointers and a segment
+//    floor for this (wf_handle::hzdp, wf_queue::I, wfqueue_hip_opt.hpp:
+//    161-178) and never uses them; its pool runs dry (:217-220).  Here:
+//     - each wave publishes a floor `lo`: every cell it may touch in its
+//       current operation has id >= lo.  It is set to cand * S, where cand
+//       is the last reclaim round's min(H, T) / S, read at the operation's
+//       start (tickets only grow, so every later ticket of either counter is
+//       above it).  The first publication of a wave in a kernel completes
+//       (s_waitcnt) before its first ticket; later ones only raise it.  A
+//       helper about to scan an older request's cells lowers it to that
+//       request's id, waits for the store, and then backs off if a reclaim
+//       has already promised those segments away (cand above them);
+//     - a reclaim round (one wave at a time, under a lock) reads H then T,
+//       publishes cand = min(H, T) / S, waits, scans the waves' floors, and
+//       frees the segments below min(cand, floors): it clears their cells
+//       and only then raises `avail`, the first logical segment not yet
+//       usable (the freed physical segments now carry the next lap);
+//     - a ticket in a segment at or above avail waits for a round to free
+//       room (boundedly: past kWindowSpins checks the operation fails, as an
+//       exhausted pool does in the reference), and a wave whose ticket
+//       reaches `next_try` starts a round.
 #pragma once
 #include "prim.hpp"
 
@@ -58,7 +81,35 @@ struct Geometry {
   uint32_t off_t, off_h, magic, pad;
   uint64_t ext_cells;   // 0: cells follow the header; else their own allocation
   uint64_t ext_handles; // handle array, for callers that pass only their own handle
+  uint32_t ring;        // 1: the pool is a ring of segments, reclaimed as the run goes
+  uint32_t seg_log2;    // ring: log2 of the cells per segment
+  uint32_t off_win, off_bound, off_lock, pad2;  // off_bound: the clearing job
 };
+
+// Ring mode's window line, read at the start of every operation (it changes
+// once or twice per reclaim round).  All values are logical segment numbers.
+struct Window {
+  uint64_t avail;     // segments below this are usable
+  uint64_t next_try;  // a ticket in this segment or above starts a reclaim round
+  uint64_t cand;      // the last round's min(H, T) / S
+  uint64_t pad[13];
+};
+static_assert(sizeof(Window) == kLine, "one line");
+constexpr uint64_t kIdle = ~0ull;            // a wave floor that holds nothing
+constexpr uint64_t kWithdrawn = (1ull << 63) - 1;  // enqueue request given up (pool exhausted)
+constexpr uint32_t kWindowSpins = 1u << 16;  // checks of `avail` before a ticket gives up
+constexpr uint32_t kPatientRounds = 16;
+constexpr uint32_t kChainRounds = 4;
+constexpr uint32_t kClearPerLane = 32;     // cells per active lane in a clearing chunk (2,048 for a full wave)
+constexpr uint32_t kClearChunkLog2 = 11;  // a full wave's chunk, for sizing a round's share
+// Segments one reclaim round may free (its clearing job): a quarter of the
+// ring, at most GCQ_YMC_ROUND_SEGS.
+#ifndef GCQ_YMC_ROUND_SEGS
+#define GCQ_YMC_ROUND_SEGS 512
+#endif
+__host__ __device__ constexpr uint64_t kRoundSegs(uint64_t P) {
+  return P / 4 > GCQ_YMC_ROUND_SEGS ? GCQ_YMC_ROUND_SEGS : (P / 4 ? P / 4 : 1);
+}
 constexpr uint32_t kMagic = 0x594D4331u;  // "YMC1"
 struct Queue {
   Geometry g;
@@ -73,6 +124,38 @@ __device__ __forceinline__ Cell* cells(Queue* q) {
 }
 __device__ __forceinline__ uint64_t* Tp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_t); }
 __device__ __forceinline__ uint64_t* Hp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_h); }
+__device__ __forceinline__ Window* win(Queue* q) { return (Window*)((char*)q + q->g.off_win); }
+// Ring mode's clearing job: the dead cells [0, end) (logical ids) are
+// cleared in chunks claimed at `next`; `cleared` counts the cleared cells.
+// A new job (a higher end) is published only once cleared == end.
+struct ClearJob {
+  uint64_t end, next, cleared, pad[13];
+};
+static_assert(sizeof(ClearJob) == kLine, "one line");
+__device__ __forceinline__ ClearJob* job(Queue* q) { return (ClearJob*)((char*)q + q->g.off_bound); }
+__device__ __forceinline__ uint32_t* lockp(Queue* q) { return (uint32_t*)((char*)q + q->g.off_lock); }
+// Ring mode: one published floor per wave (thread id / 64), after the handles.
+__host__ __device__ constexpr uint64_t wave_slots(uint32_t nthreads) { return (nthreads + 63u) / 64u; }
+__device__ __forceinline__ uint64_t* wave_lo(Queue* q, Handle* hs) { return (uint64_t*)(hs + q->g.nthreads); }
+// Per wave of a workgroup, in LDS: whether the wave has published its floor
+// in this kernel, and the value it published (0 after a helper lowered it).
+// avail and next_try are cached here too and read again every
+// kWindowRefresh operations (or when a ticket reaches the cached value): a
+// stale one only makes a ticket look the line up again.  cand is read on
+// every operation: a wave's floor must stay within reach of the counters,
+// or a dequeuer waiting at the window's end would hold the window back
+// itself.
+constexpr uint32_t kWindowRefresh = 16;
+struct WaveLds {
+  uint64_t lo[16];
+  uint64_t avail[16], next_try[16];
+  uint64_t last_t[16];  // the wave's last T ticket: a lower bound of T
+  uint8_t pub[16], age[16], low[16];
+};
+__device__ __forceinline__ WaveLds* wave_lds() {
+  __shared__ WaveLds s_wl;
+  return &s_wl;
+}
 
 __host__ __device__ __forceinline__ uint64_t pack_state(uint32_t pending, uint64_t id) {
   return ((uint64_t)pending << 63) | (id & ((1ull << 63) - 1));
@@ -113,9 +196,43 @@ __device__ inline void init_slice(Queue* q, Handle* h, uint64_t ncells, uint32_t
     g.pad = 0;
     g.ext_cells = 0;
     g.ext_handles = 0;
+    g.ring = 0;
+    g.seg_log2 = 0;
+    g.off_win = 9 * kLine;
+    g.off_bound = 11 * kLine;
+    g.off_lock = 13 * kLine;
+    g.pad2 = 0;
     q->g = g;
     *(uint64_t*)((char*)q + g.off_t) = 0;
     *(uint64_t*)((char*)q + g.off_h) = 0;
+  }
+}
+
+// Ring mode on top of init_slice (ncells a power of two, >= 4096; the cells
+// zero; the handle allocation followed by wave_slots(nthreads) floors).
+__host__ __device__ constexpr uint32_t ring_seg_log2(uint64_t ncells) {
+  uint32_t l = 0;
+  while ((1ull << (l + 1)) <= ncells / 64 && l < 10) ++l;  // S = min(1024, cells / 64): at least 64 segments
+  return l;
+}
+__device__ inline void init_ring_slice(Queue* q, Handle* hs, uint64_t ncells, uint32_t nthreads, uint32_t gid,
+                                       uint32_t stride) {
+  uint64_t* lo = (uint64_t*)(hs + nthreads);
+  for (uint64_t i = gid; i < wave_slots(nthreads); i += stride) lo[i] = kIdle;
+  if (gid == 0) {
+    const uint32_t sl = ring_seg_log2(ncells);
+    q->g.ring = 1;
+    q->g.seg_log2 = sl;
+    Window* w = (Window*)((char*)q + q->g.off_win);
+    const uint64_t P = ncells >> sl;
+    w->avail = P;
+    w->next_try = kRoundSegs(P) / 2;
+    w->cand = 0;
+    ClearJob* j = (ClearJob*)((char*)q + q->g.off_bound);
+    j->end = 0;
+    j->next = 0;
+    j->cleared = 0;
+    *(uint32_t*)((char*)q + q->g.off_lock) = 0;
   }
 }
 
@@ -127,8 +244,242 @@ struct Ops {
   uint32_t self;
   uint64_t ncells;
   Probe* pr;
+  // ring mode, loaded by prologue()
+  uint32_t ring = 0, seg_log2 = 0;
+  uint64_t avail = 0, next_try = 0, cand = 0;
 
-  __device__ __forceinline__ Cell* cell(uint64_t i) const { return i < ncells ? &cells(q)[i] : nullptr; }
+  // patient: a dequeuer's own ticket.  A dequeuer that gave its ticket up
+  // could orphan the item its enqueuer writes into that cell later, so it
+  // waits kPatientRounds times longer than an enqueuer (which may give its
+  // ticket up: the cell's dequeuer then marks the cell).
+  __device__ __forceinline__ Cell* cell(uint64_t i, bool patient = false) {
+    if (!ring) return i < ncells ? &cells(q)[i] : nullptr;
+    if ((i >> seg_log2) >= avail && !wait_window(i >> seg_log2, patient ? kPatientRounds : 1u)) return nullptr;
+    return &cells(q)[i & (ncells - 1)];
+  }
+
+  // ---------------------------------------------------------------- ring mode
+  // Start of an operation, all active lanes: load the window line; a wave's
+  // first operation in a kernel publishes its floor and waits for the store
+  // to complete before it takes any ticket.
+  __device__ __forceinline__ void prologue() {
+    ring = q->g.ring;
+    if (!ring) return;
+    seg_log2 = q->g.seg_log2;
+    const uint32_t L = first_lane(active_mask());
+    uint64_t a = 0, nt = 0, cd = 0;
+    if (lane_id() == L) {
+      WaveLds* s = wave_lds();
+      const uint32_t wv = wave_in_block();
+      const bool first = !(s->pub[wv] & 1u);
+      Window* w = win(q);
+      cd = ld_rlx(&w->cand);
+      if (first || s->age[wv] == 0) {
+        a = ld_rlx(&w->avail);
+        nt = ld_rlx(&w->next_try);
+        s->avail[wv] = a;
+        s->next_try[wv] = nt;
+        s->age[wv] = (uint8_t)kWindowRefresh;
+      } else {
+        a = s->avail[wv];
+        nt = s->next_try[wv];
+        s->age[wv] = (uint8_t)(s->age[wv] - 1u);
+      }
+      if (first) {
+        const uint64_t lo = cd << seg_log2;
+        st_rlx(&wave_lo(q, hs)[self / kWave], lo);
+        vm_wait();  // published before this wave's first ticket
+        s->lo[wv] = lo;
+        s->low[wv] = 0;
+        s->last_t[wv] = lo;
+        s->pub[wv] = 1;
+      }
+    }
+    avail = bcast64(a, L);
+    next_try = bcast64(nt, L);
+    // cand stays in the leader lane, which alone uses it (at the epilogue):
+    // no broadcast, so the operation does not wait for the load.
+    cand = cd;
+  }
+  // End of an operation, all active lanes: the wave's floor for its next
+  // operation is the cand read at this one's start (every later ticket of
+  // either counter is at least cand * S).
+  __device__ __forceinline__ void epilogue() {
+    if (!ring) return;
+    const uint32_t L = first_lane(active_mask());
+    if (lane_id() == L) {
+      WaveLds* s = wave_lds();
+      const uint32_t wv = wave_in_block();
+      const uint64_t lo = cand << seg_log2;
+      if (s->lo[wv] != lo || s->low[wv]) {
+        st_rlx(&wave_lo(q, hs)[self / kWave], lo);
+        s->lo[wv] = lo;
+        s->low[wv] = 0;
+      }
+    }
+  }
+  // A helper about to touch the cells of a request that starts at cell x:
+  // lower the wave's floor to x and wait for the store; false when a reclaim
+  // round has already promised x's segment away (cand above it), so the
+  // helper must leave that request alone.
+  __device__ __forceinline__ bool protect(uint64_t x) {
+    if (!ring) return true;
+    WaveLds* s = wave_lds();
+    const uint32_t wv = wave_in_block();
+    // x at or above the wave's published floor, or (in the lane that read
+    // it at this operation's start) within reach of cand, is covered
+    if ((x >> seg_log2) >= cand || x >= __hip_atomic_load(&s->lo[wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return true;
+    __hip_atomic_fetch_min(&wave_lo(q, hs)[self / kWave], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_min(&s->lo[wv], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    s->low[wv] = 1;  // the epilogue raises the floor again
+    vm_wait();
+    return (x >> seg_log2) >= ld_rlx(&win(q)->cand);
+  }
+  // Wait (boundedly) until segment seg is usable; lanes may be divergent.
+  __device__ bool wait_window(uint64_t seg, uint32_t rounds = 1) {
+    for (uint64_t k = 0; k < (uint64_t)kWindowSpins * rounds; ++k) {
+      const uint64_t a = ld_rlx(&win(q)->avail);
+      if (seg < a) {
+        avail = a;
+        wave_lds()->avail[wave_in_block()] = a;  // any lane: every value read is a valid lower bound
+        return true;
+      }
+      if ((k & 63u) == 0 || help_clear(1)) reclaim();
+      __builtin_amdgcn_s_sleep(2);
+    }
+    return false;
+  }
+  // After a wave-level ticket: start a round when the wave's tickets reach
+  // next_try (all active lanes).
+  __device__ __forceinline__ void maybe_reclaim(uint64_t ticket) {
+    if (!ring || !__ballot((ticket >> seg_log2) >= next_try)) return;
+    const uint32_t L = first_lane(active_mask());
+    uint64_t nt = 0;
+    if (lane_id() == L) {  // the cached next_try may be stale
+      nt = ld_rlx(&win(q)->next_try);
+      wave_lds()->next_try[wave_in_block()] = nt;
+    }
+    next_try = bcast64(nt, L);
+    if (__ballot((ticket >> seg_log2) >= next_try)) reclaim_and_clear(kRoundSegs(ncells >> seg_log2) << seg_log2 >> kClearChunkLog2);
+  }
+  // One reclaim round by the calling wave's active lanes, if the lock is free
+  // and the last clearing job is finished: publish cand, read the floors,
+  // and publish the dead range as the next clearing job (clr_end), then
+  // help clear it.
+  __device__ void reclaim() {
+    const uint64_t m = active_mask();
+    const uint32_t L = first_lane(m), n = (uint32_t)__builtin_popcountll(m), r = rank_in(m);
+    uint32_t won = 0;
+    if (lane_id() == L && ld_rlx(lockp(q)) == 0u) {
+      uint32_t e = 0;
+      won = cas_rlx(lockp(q), e, 1u) ? 1u : 0u;
+    }
+    if (!bcast32(won, L)) return;
+    Window* w = win(q);
+    ClearJob* j = job(q);
+    const uint64_t P = ncells >> seg_log2;
+    uint64_t b = 0, old_end = 0;
+    uint32_t busy = 0;
+    if (lane_id() == L) {
+      const uint64_t hh = ld_rlx(Hp(q));
+      vm_wait();  // H read before T
+      const uint64_t tt = ld_rlx(Tp(q));
+      b = (hh < tt ? hh : tt) >> seg_log2;
+      if (b > ld_rlx(&w->cand)) st_rlx(&w->cand, b);
+      vm_wait();  // cand published before the floors are read
+      old_end = ld_rlx(&j->end);
+      busy = ld_rlx(&j->cleared) != old_end;  // the last job is still being cleared
+    }
+    b = bcast64(b, L);
+    old_end = bcast64(old_end, L) >> seg_log2;
+    uint64_t nb = old_end;
+    if (!bcast32(busy, L)) {
+      const uint64_t* lo = wave_lo(q, hs);
+      uint64_t mn = kIdle;
+      for (uint64_t k = r; k < wave_slots(q->g.nthreads); k += n) {
+        const uint64_t v = ld_rlx(&lo[k]);
+        mn = v < mn ? v : mn;
+      }
+      for (uint64_t bits = m; bits; bits &= bits - 1) {
+        const uint64_t v = bcast64(mn, first_lane(bits));
+        mn = v < mn ? v : mn;
+      }
+      nb = mn == kIdle ? b : (mn >> seg_log2);
+      if (nb > b) nb = b;
+      if (nb > old_end + kRoundSegs(P)) nb = old_end + kRoundSegs(P);
+      // The segments [old_end, nb) are dead: they become the clearing job.
+      if (lane_id() == L && nb > old_end) st_rlx(&j->end, nb << seg_log2);
+    }
+    if (lane_id() == L) {
+      // the next round once the tickets have moved half a round on
+      const uint64_t step = kRoundSegs(P) / 2 ? kRoundSegs(P) / 2 : 1;
+      const uint64_t t1 = nb + step, t2 = b + step;
+      st_rlx(&w->next_try, t1 > t2 ? t1 : t2);
+      vm_wait();
+      st_rlx(lockp(q), 0u);
+    }
+  }
+  // A reclaim round, then help clear its job; a wave that finishes a job
+  // starts the next round at once (at most kChainRounds times).
+  __device__ void reclaim_and_clear(uint64_t chunks) {
+    for (uint32_t k = 0; k < kChainRounds; ++k) {
+      reclaim();
+      if (!help_clear(chunks)) return;  // no job finished: done for now
+    }
+  }
+
+  // Clear up to `chunks` chunks of the current job (any active lanes).  A
+  // chunk is claimed by a CAS on the job's cursor, cleared with device-scope
+  // stores (like every other access to a cell: clearing with plain stores
+  // and a release fence lost items, 58 in eight runs of 10^9 operations, 4
+  // with an added invalidate, because a stale copy of a cleared line in
+  // some XCD's L2 outlived the clear), and counted.  The chunk that brings
+  // the count to the job's end completes the job: every dead cell is clear,
+  // and the window moves to end / S + P.
+  __device__ bool help_clear(uint64_t chunks) {
+    const uint64_t m = active_mask();
+    const uint32_t L = first_lane(m), n = (uint32_t)__builtin_popcountll(m), r = rank_in(m);
+    ClearJob* j = job(q);
+    uint64_t* wd = (uint64_t*)cells(q);
+    for (uint64_t t = 0; t < chunks; ++t) {
+      uint64_t c0 = 0, c1 = 0, e = 0;
+      if (lane_id() == L) {
+        e = ld_rlx(&j->end);  // fixed until this job is finished, so until this chunk is counted
+        uint64_t c = ld_rlx(&j->next);
+        // kClearPerLane cells per active lane: a wave with few lanes left
+        // (a divergent waiter) takes a small chunk, so no chunk holds the
+        // job back for long.
+        const uint64_t chunk = (uint64_t)kClearPerLane * n;
+        while (c < e) {
+          const uint64_t nc = c + chunk < e ? c + chunk : e;
+          if (cas_rlx(&j->next, c, nc)) {
+            c0 = c;
+            c1 = nc;
+            break;
+          }
+        }
+      }
+      c0 = bcast64(c0, L);
+      c1 = bcast64(c1, L);
+      if (c1 == c0) return false;
+      for (uint64_t c = c0 + r; c < c1; c += n) {
+        st_rlx(&wd[2 * (c & (ncells - 1))], 0ull);
+        st_rlx(&wd[2 * (c & (ncells - 1)) + 1], 0ull);
+      }
+      vm_wait();  // this chunk is clear before it is counted
+      uint32_t fin = 0;
+      if (lane_id() == L) {
+        const uint64_t done = faa_rlx(&j->cleared, c1 - c0) + (c1 - c0);
+        if (done == e) {
+          max_rlx(&win(q)->avail, (e >> seg_log2) + (ncells >> seg_log2));
+          fin = 1;
+        }
+      }
+      if (bcast32(fin, L)) return true;
+    }
+    return false;
+  }
 
   __device__ __forceinline__ uint32_t ref_of(uint32_t tid) const { return tid + 2u; }
   __device__ __forceinline__ Handle* handle_of(uint32_t ref) const { return &hs[ref - 2u]; }
@@ -156,9 +507,14 @@ struct Ops {
     const uint64_t i = ct ? group_faa(ct, Tp(q), (uint64_t)1) : wave_faa(Tp(q), (uint64_t)1);
     pr->step();
     pr->after_ticket();
+    if (ring && lane_id() == first_lane(active_mask())) wave_lds()->last_t[wave_in_block()] = i;
+    maybe_reclaim(i);
     Cell* c = cell(i);
     *cid = i;
-    if (!c) return false;
+    if (!c) {
+      *cid = kWithdrawn;  // no cell: the pool is exhausted
+      return false;
+    }
     return cas_once(&c->val, kBottom, v);
   }
 
@@ -171,7 +527,12 @@ struct Ops {
       pr->step();
       const uint64_t i = faa_rlx(Tp(q), (uint64_t)1);
       Cell* c = cell(i);
-      if (!c) break;
+      if (!c) {
+        // Pool exhausted: withdraw the request unless a helper has already
+        // claimed a cell for it (then that cell receives the value below).
+        if (cas_once(&h->enq_state, pack_state(1, cell_id), pack_state(0, kWithdrawn))) return false;
+        break;
+      }
       if (cas_once(&c->enq, kRefNull, ref_of(self))) {
         const uint64_t cur = ld_rlx(&c->val);
         if (cur == kBottom || cur == kTop) {
@@ -184,7 +545,15 @@ struct Ops {
     const uint64_t fid = st_id(ld_rlx(&h->enq_state));
     Cell* d = cell(fid);
     if (!d) return false;  // pool exhausted
-    if (ld_rlx(&d->val) == kTop) enq_commit(d, v, fid);
+    // The enqueuer writes its value into the cell claimed for its request
+    // whatever the cell holds (⊥ or ⊤), as YMC's enq_slow does.  The
+    // reference writes it only over ⊤ (wfqueue_hip_opt.hpp:308-314) and
+    // otherwise leaves it to the cell's dequeuer, which reads the value from
+    // the request; once this thread has announced its next slow-path request
+    // that value is gone, and the item is lost.  No other enqueuer holds this
+    // cell's ticket, and a helper commits the same value, so the store is
+    // safe.
+    enq_commit(d, v, fid);
     return true;
   }
 
@@ -261,13 +630,14 @@ struct Ops {
         const uint64_t h0 = ld_rlx(Hp(q));
         vm_wait();  // H is read before T
         const uint64_t t0 = ld_rlx(Tp(q));
-        const uint64_t avail = t0 > h0 ? t0 - h0 : 0;
-        k = avail < n ? (uint32_t)avail : n;
+        const uint64_t room = t0 > h0 ? t0 - h0 : 0;
+        k = room < n ? (uint32_t)room : n;
         if (k) base = faa_rlx(Hp(q), (uint64_t)k);
       }
       k = bcast32(k, leader);
       if (k == 0) return false;
       base = bcast64(base, leader);
+      maybe_reclaim(base);
       const uint32_t rank = rank_in(m);
       if (rank < k) {
         *ticket = base + rank;
@@ -279,7 +649,7 @@ struct Ops {
   __device__ uint64_t deq_fast(uint64_t i) {  // :416-434, ticket from claim_h
     pr->step();
     pr->after_ticket();
-    Cell* c = cell(i);
+    Cell* c = cell(i, true);
     if (!c) return kEmpty;
     // The ticket lies below T: its enqueuer is in flight.  Give it a moment
     // before help_enq marks the cell TOP and sends the enqueuer round again.
@@ -299,6 +669,7 @@ struct Ops {
     uint64_t s = ld_rlx(&helpee->deq_state);
     const uint64_t id = ld_rlx(&helpee->deq_id);
     if (!st_pending(s) || st_id(s) < id) return;
+    if (helpee != h && !protect(id)) return;  // ring mode: the request's cells may be reclaimed
     uint64_t prior = id, i = id, cand = 0;
     for (;;) {
       pr->step();
@@ -350,16 +721,73 @@ struct Ops {
     return v == kTop ? kEmpty : v;
   }
 
-  __device__ bool enqueue(uint64_t v, Combine* ct = nullptr) {  // :527-536
+  // Ring mode, all active lanes, before an enqueue takes a ticket: false
+  // when T is within kFullMargin segments of the window's end (the queue is
+  // full).  A producer that took a ticket beyond the window would have to
+  // wait for room there, and dequeuers that reach its cell would wait for
+  // it, holding the window back themselves; refusing the enqueue instead
+  // keeps every ticket inside the window (up to what other waves take
+  // between the check and their FAA, which the margin covers).  The check
+  // costs nothing until the wave's last T ticket comes near the end.
+  __device__ bool room_to_enqueue() {
+    if (!ring) return true;
+    const uint64_t margin = (q->g.nthreads >> seg_log2) + 2;
+    const uint32_t L = first_lane(active_mask());
+    const uint64_t a0 = avail;  // wave-uniform
+    uint64_t a = a0;
+    uint32_t near = 0;
+    if (lane_id() == L) {
+      WaveLds* s = wave_lds();
+      near = (s->last_t[wave_in_block()] >> seg_log2) + margin >= a0;
+    }
+    if (!bcast32(near, L)) return true;
+    uint32_t ok = 0;
+    for (int k = 0; k < 2; ++k) {
+      if (lane_id() == L) {
+        WaveLds* s = wave_lds();
+        const uint32_t wv = wave_in_block();
+        const uint64_t t = ld_rlx(Tp(q));
+        a = ld_rlx(&win(q)->avail);
+        s->avail[wv] = a;
+        s->last_t[wv] = t;
+        ok = (t >> seg_log2) + margin < a;
+      }
+      a = bcast64(a, L);
+      ok = bcast32(ok, L);
+      if (ok || k) break;
+      // Try once to move the window: a round, and help clear its job (or the
+      // unfinished one); no ticket is taken while the queue is full, so no
+      // ticket-driven round would come.
+      reclaim_and_clear(kRoundSegs(ncells >> seg_log2) << seg_log2 >> kClearChunkLog2);
+    }
+    avail = a;
+    return ok != 0;
+  }
+
+  __device__ bool enqueue_body(uint64_t v, Combine* ct) {  // :527-536
+    if (!room_to_enqueue()) return false;  // ring mode: full
     for (int p = (int)q->g.patience; p >= 0; --p) {
       uint64_t cid = 0;
       if (enq_fast(v, &cid, ct)) return true;
+      if (cid == kWithdrawn) return false;  // pool exhausted
       if (p == 0) return enq_slow(v, cid);
     }
     return false;
   }
+  __device__ bool enqueue(uint64_t v, Combine* ct = nullptr) {
+    prologue();
+    const bool ok = enqueue_body(v, ct);
+    epilogue();
+    return ok;
+  }
+  __device__ uint64_t dequeue() {
+    prologue();
+    const uint64_t v = dequeue_body();
+    epilogue();
+    return v;
+  }
 
-  __device__ uint64_t dequeue() {  // :538-555
+  __device__ uint64_t dequeue_body() {  // :538-555
     // Every ticket comes from claim_h, which takes no cell when T <= H: then
     // every enqueue cell is already owned by a dequeuer (EMPTY at the T
     // read).  Idle consumers would otherwise burn a cell per attempt.
@@ -393,6 +821,25 @@ __device__ __forceinline__ uint64_t dequeue(Queue* q, Handle* hs, uint32_t tid, 
   Ops<Probe> o{q, hs, &hs[tid], tid, q->g.cells, pr ? pr : (Probe*)&np};
   return o.dequeue();
 }
+// Ring mode: every wave of a kernel that operates on the queue calls
+// begin() before its first operation and end() after its last (any active
+// lanes; lanes that run no operation may call them too).  begin() marks the
+// wave's floor unpublished in LDS; end() withdraws a published floor, so a
+// wave that stops using the queue does not hold reclamation back.
+__device__ __forceinline__ void begin(Queue* q) {
+  if (!q->g.ring) return;
+  if (lane_id() == first_lane(active_mask())) wave_lds()->pub[wave_in_block()] = 0;
+}
+__device__ __forceinline__ void end(Queue* q, Handle* hs, uint32_t tid) {
+  if (!q->g.ring) return;
+  if (lane_id() == first_lane(active_mask())) {
+    WaveLds* s = wave_lds();
+    const uint32_t wv = wave_in_block();
+    if (s->pub[wv] & 1u) st_rlx(&wave_lo(q, hs)[tid / kWave], kIdle);
+    s->pub[wv] = 0;
+  }
+}
+
 // High-water mark of touched cells (for resets and pool-exhaustion checks).
 __device__ __forceinline__ uint64_t high_water(Queue* q) {
   const uint64_t t = ld_rlx(Tp(q)), h = ld_rlx(Hp(q));

@@ -27,8 +27,16 @@ namespace gcqk {
 // context for drawing the producers' Tail tickets together (group_faa).
 // G-WFQ and G-LFQ use both, WFQ only the second; SFQ ignores them.
 
+// begin(tid) / end(tid): called by every wave of a kernel before its first
+// and after its last operation on the queue (WFQ's ring mode publishes and
+// withdraws a floor per wave; the other queues need nothing).
+#define GCQ_NO_LIFETIME                                  \
+  __device__ __forceinline__ void begin(uint32_t) const {} \
+  __device__ __forceinline__ void end(uint32_t) const {}
+
 struct GwfQ {
   gcq::gwf::Mpmc m;
+  GCQ_NO_LIFETIME
   template <class P = gcq::gwf::NoProbe>
   __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
     return gcq::gwf::enqueue_ptr<P>(m, tid, v, p, ct);
@@ -47,6 +55,7 @@ struct GwfQ {
 
 struct LfqQ {
   gcq::lfq::Mpmc m;
+  GCQ_NO_LIFETIME
   template <class P = gcq::lfq::NoProbe>
   __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
     return gcq::lfq::enqueue_ptr<P>(m, v, p, ct);
@@ -81,6 +90,7 @@ struct SfqQ {
   gcq::sfq::Queue* q;
   uint32_t exact_empty;
   uint32_t blocking;
+  GCQ_NO_LIFETIME
   template <class P = gcq::sfq::NoProbe>
   __device__ __forceinline__ bool enq(uint32_t, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
     if (blocking) return gcq::sfq::enqueue_blocking(q, v, kBlockingSpins, p) == gcq::sfq::kSuccess;
@@ -103,6 +113,8 @@ struct SfqQ {
 struct YmcQ {
   gcq::ymc::Queue* q;
   gcq::ymc::Handle* h;
+  __device__ __forceinline__ void begin(uint32_t) const { gcq::ymc::begin(q); }
+  __device__ __forceinline__ void end(uint32_t tid) const { gcq::ymc::end(q, h, tid); }
   template <class P = gcq::ymc::NoProbe>
   __device__ __forceinline__ bool enq(uint32_t tid, uint64_t v, P* p = nullptr, gcq::Combine* ct = nullptr) const {
     return gcq::ymc::enqueue(q, h, tid, v, p, ct);

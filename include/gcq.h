@@ -51,7 +51,10 @@ typedef struct gcq_queue_config {
   uint32_t help_delay;    /* G-WFQ: ops between help turns (ref WF_HELP_DELAY 64) */
   uint32_t enq_patience;  /* G-WFQ fast-path attempts + 1 (ref WF_ENQ_PATIENCE 128) */
   uint32_t deq_patience;  /* (ref WF_DEQ_PATIENCE 256) */
-  uint32_t wfq_segment_cells; /* WFQ: cells reserved in the segment pool (0 = auto) */
+  uint32_t wfq_segment_cells; /* WFQ: cells in the pool, a power of two >= 4096; the pool is a ring of
+                                 segments reclaimed while the queue runs, and an enqueue is refused (full)
+                                 when the ring has no room (0 = max(capacity, 64 per thread) rounded
+                                 up to a power of two) */
   uint32_t poll_backoff;  /* G-WFQ/G-LFQ consumer polite-polling base, s_sleep units (0 = off) */
 } gcq_queue_config;
 

@@ -601,7 +601,10 @@ struct Ymc {
     const u64 fid = sid(h.enq_state.load());
     Cell* d = cell(fid);
     if (!d) return false;
-    if (d->val.load() == TOP) commit(d, v, fid);
+    // YMC's enq_slow commits its value whatever the claimed cell holds; the
+    // reference commits only over TOP (wfqueue_hip_opt.hpp:308-314), which
+    // loses the item once the handle's next request replaces the value.
+    commit(d, v, fid);
     return true;
   }
   u64 help_enq(u32 me, Cell* c, u64 i) {

@@ -108,8 +108,7 @@ def test_grouped_tail_tickets_conserve_items(gcq, kind, mode):
     # 32,768 threads: the smallest grid in which the throughput kernel draws
     # producers' Tail tickets per workgroup (the headline kernel's form).
     print(f"[tb grouped] kind={NAMES[kind]} mode={mode}", flush=True)
-    kw = {"wfq_segment_cells": 1 << 26} if kind == 2 else {}
-    with gcq.Queue(kind, capacity=1 << 20, max_threads=32768, poll_backoff=0, **kw) as q:
+    with gcq.Queue(kind, capacity=1 << 20, max_threads=32768, poll_backoff=0) as q:
         r = q.tb_run(mode, threads=32768, chunk_ops=32, producer_percent=25, warmup_launches=2, timed_launches=5)
         print(f"[tb grouped] succ {r['succ_mops']:.1f} Mops/s, enq {r['enq_success']} deq {r['deq_success']}",
               flush=True)
@@ -213,3 +212,24 @@ def test_remap_history_and_conservation(gcq, orc, kind):
         r = q.tb_run(1, threads=8192, chunk_ops=32, producer_percent=25, warmup_launches=3, timed_launches=20)
         assert r["enq_success"] > 0 and r["deq_success"] > 0
         _conserved(q, r, f"remap {NAMES[kind]}")
+
+
+@pytest.mark.parametrize("mode", [1, 0], ids=["split25", "balanced"])
+def test_wfq_bounded_pool_ms_window(gcq, orc, mode):
+    # WFQ on its default pool at capacity 2^20 and 65,536 threads: a ring of
+    # 2^22 cells (64 MiB), reclaimed while the queue runs.  128 ops per
+    # launch, a 200 ms warmup and a 1,000 ms timed window: about 2 * 10^9
+    # cell tickets through 2^22 cells.  No enqueue may be refused, and every item
+    # is accounted for.
+    print(f"[wfq ring] mode={mode}", flush=True)
+    with gcq.Queue(2, capacity=1 << 20, max_threads=65536) as q:
+        assert q.cfg.wfq_segment_cells == 0  # the library's default pool
+        r = q.tb_run(mode, threads=65536, chunk_ops=128, producer_percent=25, warmup_ms=200, run_ms=1000)
+        print(f"[wfq ring] {r['succ_mops']:.1f} Mops/s, launches {r['launches']}", flush=True)
+        assert r["enq_fail"] == 0
+        assert r["succ_mops"] > 100
+        _conserved(q, r, f"wfq ring mode={mode}")
+        # the same queue, after ~10^9 tickets, still gives linearizable histories
+        h = q.hist_run(threads=65536, ops=4, block=256, mode=0)
+    assert len(h) == 65536 * 4 * 2
+    _check(orc, h, "wfq ring after a long run")
