@@ -67,17 +67,23 @@ def _profile_traffic(name):
 QUEUE_NAMES = {0: "G-WFQ", 1: "G-LFQ", 2: "WFQ", 3: "SFQ"}
 
 
+# The CPU set at start: the OpenMP oracle binds the thread that loads it to
+# one core, so the affinity read later can be a single CPU.
+_START_MASK = frozenset(os.sched_getaffinity(0))
+
+
 def host_cores():
     """The host share of one GPU is 16 cores, whatever os.cpu_count() reports."""
-    return min(len(os.sched_getaffinity(0)), 16)
+    return min(len(_START_MASK), 16)
 
 
 def cpu_baseline(kind, cores, run_ms, warmup_ms=200):
     """The oracle's host std::atomic restatement of the queue, split 25/75,
-    swept over 1, 2, 4, ... host threads up to the host share: value is the
+    swept over 2, 4, ... host threads up to the host share: value is the
     best rate (cores = the threads that gave it), beside the all-cores rate."""
     from oracle import pyoracle as orc
-    counts = sorted({1 << i for i in range(cores.bit_length()) if (1 << i) <= cores} | {cores})
+    # from 2 threads: the split harness makes a lone thread a producer with no consumer
+    counts = sorted({1 << i for i in range(1, cores.bit_length()) if (1 << i) <= cores} | {max(cores, 2)})
     sweep = {}
     for n in counts:
         r = orc.tb_run(kind, threads=n, mode=1, producer_percent=25, chunk_ops=128, warmup_ms=warmup_ms,
@@ -85,7 +91,7 @@ def cpu_baseline(kind, cores, run_ms, warmup_ms=200):
         sweep[n] = r["succ_mops"]
     best = max(sweep, key=sweep.get)
     return {"value": round(sweep[best], 3), "unit": "Mops/s", "cores": best, "kind": "port",
-            "all_cores": {"value": round(sweep[cores], 3), "cores": cores},
+            "all_cores": {"value": round(sweep[max(cores, 2)], 3), "cores": max(cores, 2)},
             "sweep": {str(n): round(v, 3) for n, v in sweep.items()},
             "sample": f"host std::atomic {QUEUE_NAMES[kind]} restatement, split 25/75, swept over "
                       f"{', '.join(map(str, counts))} threads, chunk 128, {warmup_ms} ms warmup + {run_ms} ms timed "
@@ -307,15 +313,15 @@ def secondary(g, with_cpu):
     names = {g.GWFQ: "gwfq", g.GLFQ: "glfq", g.WFQ: "wfq", g.SFQ: "sfq"}
     for kind in (g.GWFQ, g.GLFQ, g.WFQ, g.SFQ):
         rows = {}
-        # WFQ's cell pool is its default: 2^20 cells, a ring reclaimed as it runs
-        extra = {}
+        # WFQ's cell pool is its default: 2^22 cells at 65,536 threads, a ring
+        # reclaimed as the queue runs
         for mode, mname in ((g.SPLIT, "split25"), (0, "balanced")):
             for T in (1024, 2048, 4096, 8192, 16384, 32768, 65536):
-                with g.Queue(kind, capacity=1 << 20, max_threads=65536, poll_backoff=0, **extra) as q:
+                with g.Queue(kind, capacity=1 << 20, max_threads=65536, poll_backoff=0) as q:
                     r = q.tb_run(mode, threads=T, block=256, chunk_ops=128, producer_percent=25,
                                  warmup_launches=5, timed_launches=20)
                 rows[f"{mname}_{T}"] = round(r["succ_mops"], 1)
-        with g.Queue(kind, capacity=1 << 20, max_threads=65536, **extra) as q:
+        with g.Queue(kind, capacity=1 << 20, max_threads=65536) as q:
             f = q.fifo_check(65536, 32, concurrent=True)
         rows["fifo_check_65536x32_verified"] = bool(f["verified"])
         if with_cpu:

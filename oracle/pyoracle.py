@@ -35,6 +35,18 @@ class TbRes(C.Structure):
 
 _lib = None
 
+# The CPU set this process started with.  liboracle.so is built with OpenMP;
+# when OMP_PROC_BIND is set, libgomp binds the thread that loads it (and the
+# master of each parallel region) to one place, and std::threads started by
+# that thread inherit its mask.  The calls that start std::threads restore the
+# full set first.
+HOST_MASK = frozenset(os.sched_getaffinity(0))
+
+
+def _unpin():
+    if os.sched_getaffinity(0) != HOST_MASK:
+        os.sched_setaffinity(0, HOST_MASK)
+
 
 def _rt_sig(lib):
     lib.orc_rt_render.restype = C.c_int
@@ -68,6 +80,7 @@ def load():
         lib.orc_bfs_parallel.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int]
         _rt_sig(lib)
         _lib = lib
+        _unpin()
     return _lib
 
 
@@ -104,7 +117,9 @@ def hist_run(kind, threads, ops, mode=0, producer_percent=50, capacity=32768, en
     cap = threads * ops * 2
     out = np.zeros(cap, dtype=OP_DTYPE)
     steps = C.c_uint32(0)
-    n = load().orc_hist_run(C.byref(cfg), out.ctypes.data, cap, C.byref(steps))
+    lib = load()
+    _unpin()
+    n = lib.orc_hist_run(C.byref(cfg), out.ctypes.data, cap, C.byref(steps))
     if n < 0:
         raise ValueError("orc_hist_run failed")
     return out[:n], steps.value
@@ -114,7 +129,9 @@ def tb_run(kind, threads, mode=1, producer_percent=25, chunk_ops=128, warmup_ms=
            capacity=1 << 20):
     cfg = TbCfg(kind, threads, mode, producer_percent, chunk_ops, warmup_ms, run_ms, capacity)
     res = TbRes()
-    if load().orc_tb_run(C.byref(cfg), C.byref(res)) != 0:
+    lib = load()
+    _unpin()
+    if lib.orc_tb_run(C.byref(cfg), C.byref(res)) != 0:
         raise ValueError("orc_tb_run failed")
     return {f: getattr(res, f) for f, _ in TbRes._fields_}
 
@@ -161,7 +178,9 @@ def rt_render(spheres, planes, cam12, light3, ambient3, W, H, x0=0, y0=0, w=0, h
     am = np.ascontiguousarray(ambient3, np.float32)
     img = np.empty((h, w, 3), np.float32)
     rays = C.c_uint64(0)
-    rc = (lib or load()).orc_rt_render(sph.ctypes.data, len(sph), pl.ctypes.data, len(pl), cam.ctypes.data, li.ctypes.data,
+    lib = lib or load()
+    _unpin()
+    rc = lib.orc_rt_render(sph.ctypes.data, len(sph), pl.ctypes.data, len(pl), cam.ctypes.data, li.ctypes.data,
                               am.ctypes.data, W, H, x0, y0, w, h, max_depth, threads, img.ctypes.data, C.byref(rays))
     if rc != 0:
         raise ValueError("orc_rt_render: bad region")
