@@ -6,6 +6,7 @@ resulting gcq_amd/libgcq.so is what every Python entry point loads.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -34,15 +35,16 @@ def sources():
     return [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
     srcs = sources()
-    objs = []
-    for s in srcs:
-        obj = os.path.join(CSRC, "_obj", os.path.splitext(s)[0] + ".o")
-        os.makedirs(os.path.dirname(obj), exist_ok=True)
-        cmd = ["hipcc", *HIPCC_FLAGS, "-c", os.path.join(CSRC, s), "-o", obj]
-        _run(cmd, verbose)
-        objs.append(obj)
+    objs = [os.path.join(CSRC, "_obj", os.path.splitext(s)[0] + ".o") for s in srcs]
+    os.makedirs(os.path.join(CSRC, "_obj"), exist_ok=True)
+    cmds = [["hipcc", *HIPCC_FLAGS, "-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(srcs, objs)]
+    # one hipcc per translation unit, at most one per core (capped at 8)
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1, len(cmds)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
