@@ -254,7 +254,7 @@ def profile_run(a):
                    frac_p1=(r["enq_success"] + r["deq_success"]) / a.steps / (ms * 1e-3) / p1["tickets_per_s"])
     elif a.profile == "bfs":
         from gcq_amd import bfs
-        mode = {"persistent": bfs.PERSISTENT, "host": bfs.HOST_LOOP, "async": bfs.ASYNC}[a.mode]
+        mode = {"persistent": bfs.PERSISTENT, "host": bfs.HOST_LOOP, "async": bfs.ASYNC, "bitmap": bfs.BITMAP}[a.mode]
         with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as gr:
             for _ in range(a.runs):
                 _, res = gr.bfs(0, kind=g.KINDS[a.queue or "glfq"], threads=8192, block=256, mode=mode, depths=False)
@@ -349,6 +349,18 @@ def secondary(g, with_cpu):
              "roofline": {"bound": "hbm", "achieved": round(alg_bytes / med / 1e6, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(alg_bytes / med / 1e6 / HBM_PEAK_GBS, 5),
                           "bytes_per_launch": alg_bytes}}
+        # the same traversal in the other one-launch modes (1 warmup + 5 timed)
+        o["other_modes"] = {}
+        for mname, mode in (("bitmap", bfs.BITMAP), ("async", bfs.ASYNC)):
+            gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=mode, depths=False)
+            mm = []
+            for _ in range(5):
+                _, r2 = gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=mode, depths=False)
+                mm.append(r2["kernel_ms"])
+            m2 = float(np.median(mm))
+            o["other_modes"][mname] = {"kernel_ms_median": round(m2, 3),
+                                       "mteps": round(res["edges_scanned"] / m2 / 1e3, 1),
+                                       "expansions_over_edges": round(r2["edges_scanned"] / res["edges_scanned"], 3)}
         if with_cpu:
             from oracle import pyoracle as orc
             off, idx = gr.csr()

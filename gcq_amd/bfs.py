@@ -9,7 +9,7 @@ import numpy as np
 from . import GLFQ, GWFQ, SFQ, WFQ, _check, load  # noqa: F401
 from .workloads import BatchConfig, BatchResult, BfsConfig, BfsResult, QueryResult, TuneTrial
 
-HOST_LOOP, PERSISTENT, ASYNC = 0, 1, 2
+HOST_LOOP, PERSISTENT, ASYNC, BITMAP = 0, 1, 2, 3
 
 
 class Graph:

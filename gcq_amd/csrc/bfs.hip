@@ -80,11 +80,20 @@ constexpr int32_t kProbeEdges = 256;
 // starting Head is not known.
 constexpr uint64_t kNoHead = ~0ull;
 
-template <bool kCounted, class F>
+// kBitmap (GCQ_BFS_BITMAP): a vertex is claimed in the visited bitmap `vis`
+// (one bit per vertex, 1,024 vertices per 128-B line) instead of by a CAS on
+// its depth word.  Each edge first reads its bit through the caches (bits
+// only ever go from 0 to 1, so a stale read can only send a visited vertex
+// on to the atomic), a clear bit is claimed by fetch-or, and the one lane
+// whose fetch-or set the bit stores the depth with a plain store: nothing
+// reads depths during the traversal, and the kernel's end makes them
+// visible.  Visited neighbours then cost a cached load instead of an atomic
+// on the memory side.
+template <bool kCounted, class F, bool kBitmap = false>
 __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restrict__ offs,
                              const int32_t* __restrict__ idx, int32_t* level, int32_t L, uint32_t tid, uint32_t* stage,
                              LevelAcc& acc, uint32_t* err, uint64_t h0 = kNoHead, uint64_t grid_lanes = 0,
-                             uint64_t* end_out = nullptr) {
+                             uint64_t* end_out = nullptr, uint32_t* vis = nullptr) {
   const uint32_t lane = lane_id();
   uint32_t sc = 0;  // staged vertices (wave-uniform)
   uint64_t end = 0;
@@ -144,9 +153,16 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
       uint32_t w = 0;
       if (e < total) {
         w = w_here;
-        // A depth changes once per traversal, from -1 to a level, so a cached
-        // (possibly stale) read can only send a visited vertex to the CAS.
-        if (!probe || gcq::ld_cached(&level[w]) == -1) {
+        if constexpr (kBitmap) {
+          uint32_t* const vw = vis + (w >> 5);
+          const uint32_t vb = 1u << (w & 31u);
+          if (!(gcq::ld_cached(vw) & vb)) {
+            disc = !(gcq::or_rlx(vw, vb) & vb);
+            if (disc) level[w] = L + 1;
+          }
+        } else if (!probe || gcq::ld_cached(&level[w]) == -1) {
+          // A depth changes once per traversal, from -1 to a level, so a cached
+          // (possibly stale) read can only send a visited vertex to the CAS.
           int32_t expect = -1;
           disc = gcq::cas_rlx(&level[w], expect, L + 1);
         }
@@ -172,9 +188,9 @@ __device__ void finish_counts(LevelAcc& acc, uint64_t disc_total, Ctl* c) {
   }
 }
 
-template <class F>
+template <class F, bool kBitmap = false>
 __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
-                             int32_t* level, Ctl* c) {
+                             int32_t* level, Ctl* c, uint32_t* vis) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
@@ -194,8 +210,8 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
       if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
     acc.disc = 0;
     uint64_t end = 0;
-    expand_level<F::kCounted>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
-                              F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads, &end);
+    expand_level<F::kCounted, F, kBitmap>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
+                                          F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads, &end, vis);
     prev_end[L & 1] = end;
     disc_total += acc.disc;
     uint32_t n = 0;
@@ -234,11 +250,13 @@ __global__ void k_rewind(Frontiers<F> fq, int32_t L) {
   if (threadIdx.x == 0) fq.idle(L).rewind();
 }
 
-// Seed the level-0 frontier with the source.
+// Seed the level-0 frontier with the source (and mark it in the visited
+// bitmap, when there is one).
 template <class F>
-__global__ void k_seed(F q, int32_t* level, int32_t src, Ctl* c) {
+__global__ void k_seed(F q, int32_t* level, int32_t src, Ctl* c, uint32_t* vis) {
   if (threadIdx.x == 0) {
     level[src] = 0;
+    if (vis) vis[src >> 5] = 1u << (src & 31);
     c->pending = 1;
     if (!q.push(0, (uint32_t)src)) c->err = 1;
   }
@@ -392,10 +410,14 @@ __global__ void k_levels(const int32_t* level, uint64_t n, Ctl* c) {
 // ======================================================================== host side
 namespace {
 
+// The visited bitmap: one bit per vertex, in whole 128-B lines.
+uint64_t vis_bytes(uint64_t nv) { return (nv + 1023) / 1024 * 128; }
+
 int ctx_free(gcq_graph* g) {
   BfsCtx& c = g->ctx;
   c.fs.release();
   if (c.level) (void)hipFree(c.level);
+  if (c.vis) (void)hipFree(c.vis);
   if (c.ctl) (void)hipFree(c.ctl);
   if (c.pinned) (void)hipHostFree(c.pinned);
   c = BfsCtx();
@@ -409,22 +431,27 @@ int ctx_ensure(gcq_graph* g, int kind, uint32_t threads) {
   if (rc) return rc;
   if (!c.level) {
     GCQ_HIP(hipMalloc(&c.level, 4ull * nv));
+    GCQ_HIP(hipMalloc(&c.vis, vis_bytes(nv)));
     GCQ_HIP(hipMalloc(&c.ctl, sizeof(bfsk::Ctl)));
     GCQ_HIP(hipHostMalloc((void**)&c.pinned, 64));
   }
   return GCQ_OK;
 }
 
-int reset_run(gcq_graph* g, int32_t src) {
+int reset_run(gcq_graph* g, int32_t src, bool bitmap) {
   BfsCtx& c = g->ctx;
   const uint32_t nv = (uint32_t)g->host.nv;
   hipLaunchKernelGGL(gcqf::kf_fill_i32, dim3(gcqf::grid_of(nv)), dim3(256), 0, g->stream, c.level, (uint64_t)nv, -1);
+  if (bitmap)
+    hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(vis_bytes(nv) / 8)), dim3(256), 0, g->stream,
+                       (uint64_t*)c.vis, vis_bytes(nv) / 8);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, g->stream, (uint64_t*)c.ctl,
                      (uint64_t)(sizeof(bfsk::Ctl) / 8));
   c.fs.reset(g->stream);
   c.fs.dispatch([&](const auto& fq) {
     using F = std::decay_t<decltype(fq.q[0])>;
-    hipLaunchKernelGGL(bfsk::k_seed<F>, dim3(1), dim3(64), 0, g->stream, fq.q[0], c.level, src, (bfsk::Ctl*)c.ctl);
+    hipLaunchKernelGGL(bfsk::k_seed<F>, dim3(1), dim3(64), 0, g->stream, fq.q[0], c.level, src, (bfsk::Ctl*)c.ctl,
+                       bitmap ? c.vis : nullptr);
     return GCQ_OK;
   });
   GCQ_HIP(hipGetLastError());
@@ -453,18 +480,24 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
       hipLaunchKernelGGL(bfsk::k_levels, dim3(gcqf::grid_of(nv)), dim3(256), 0, g->stream, c.level, (uint64_t)nv,
                          ctl);
     }
-  } else if (cfg->mode == GCQ_BFS_PERSISTENT) {
+  } else if (cfg->mode == GCQ_BFS_PERSISTENT || cfg->mode == GCQ_BFS_BITMAP) {
+    const bool bitmap = cfg->mode == GCQ_BFS_BITMAP;
     int per_cu = 0, dev = 0, cus = 0;
     GCQ_HIP(hipGetDevice(&dev));
     GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfsk::k_persistent<F>, cfg->block, 0));
+    GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, bitmap ? bfsk::k_persistent<F, true> : bfsk::k_persistent<F, false>, cfg->block, 0));
     if ((int64_t)grid > (int64_t)per_cu * cus) {
       gcqh::set_error("persistent BFS needs all %u blocks resident; the device holds %d", grid, per_cu * cus);
       rc = GCQ_EINVAL;
     } else {
       GCQ_HIP(hipEventRecord(e0, g->stream));
-      hipLaunchKernelGGL(bfsk::k_persistent<F>, dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off, g->d_idx,
-                         c.level, ctl);
+      if (bitmap)
+        hipLaunchKernelGGL((bfsk::k_persistent<F, true>), dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off,
+                           g->d_idx, c.level, ctl, c.vis);
+      else
+        hipLaunchKernelGGL((bfsk::k_persistent<F, false>), dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off,
+                           g->d_idx, c.level, ctl, (uint32_t*)nullptr);
       GCQ_HIP(hipEventRecord(e1, g->stream));
     }
   } else {
@@ -606,7 +639,8 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   GCQ_REQUIRE(cfg->block >= 64 && cfg->block % 64 == 0 && cfg->block <= 1024, "block must be a multiple of 64");
   GCQ_REQUIRE(cfg->threads >= cfg->block && cfg->threads % cfg->block == 0,
               "threads must be a positive multiple of block (whole waves, bfs_bench.cpp:521)");
-  GCQ_REQUIRE(cfg->mode == GCQ_BFS_HOST_LOOP || cfg->mode == GCQ_BFS_PERSISTENT || cfg->mode == GCQ_BFS_ASYNC,
+  GCQ_REQUIRE(cfg->mode == GCQ_BFS_HOST_LOOP || cfg->mode == GCQ_BFS_PERSISTENT || cfg->mode == GCQ_BFS_ASYNC ||
+                  cfg->mode == GCQ_BFS_BITMAP,
               "bad mode");
   std::memset(res, 0, sizeof *res);
   if (src < 0 || src >= g->host.nv) src = 0;  // bfs_bench.cpp:555
@@ -615,7 +649,7 @@ extern "C" int gcq_bfs_run(gcq_graph* g, int32_t src, const gcq_bfs_config* cfg,
   const auto t0 = std::chrono::steady_clock::now();
   rc = ctx_ensure(g, cfg->kind, cfg->threads);
   if (rc) return rc;
-  rc = reset_run(g, src);
+  rc = reset_run(g, src, cfg->mode == GCQ_BFS_BITMAP);
   if (rc) return rc;
   BfsCtx& c = g->ctx;
   double kms = 0;
@@ -718,9 +752,9 @@ extern "C" int gcq_bfs_run_noalloc(const int32_t* d_offsets, const int32_t* d_in
   hipLaunchKernelGGL(gcqf::kf_fill_i32, dim3(gcqf::grid_of(nv)), dim3(256), 0, s, d_depth, (uint64_t)nv, -1);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, s, (uint64_t*)ctl, (uint64_t)(sizeof(bfsk::Ctl) / 8));
   hipLaunchKernelGGL(gcqf::kf_reset_lfq, dim3(gcqf::grid_of(2ull * cap)), dim3(256), 0, s, ring0, ring1, cap);
-  hipLaunchKernelGGL(bfsk::k_seed<gcqf::FLfq>, dim3(1), dim3(64), 0, s, fq.q[0], d_depth, src, ctl);
+  hipLaunchKernelGGL(bfsk::k_seed<gcqf::FLfq>, dim3(1), dim3(64), 0, s, fq.q[0], d_depth, src, ctl, (uint32_t*)nullptr);
   hipLaunchKernelGGL(bfsk::k_persistent<gcqf::FLfq>, dim3(grid), dim3(block), 0, s, fq, d_offsets, d_indices, d_depth,
-                     ctl);
+                     ctl, (uint32_t*)nullptr);
   GCQ_HIP(hipGetLastError());
   bfsk::Ctl h;
   GCQ_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, s));

@@ -7,6 +7,7 @@
 struct BfsCtx {
   gcqf::FrontierSet fs;
   int32_t* level = nullptr;
+  uint32_t* vis = nullptr;  // visited bitmap (GCQ_BFS_BITMAP)
   void* ctl = nullptr;
   uint32_t* pinned = nullptr;  // host-visible counters for the host-loop mode
 };

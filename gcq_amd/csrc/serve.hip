@@ -15,13 +15,22 @@
 //    remaining blocks form an RT sub-grid: each wave picks an active tile
 //    job, claims 64 primary rays at a time or pops a live bounce ray from
 //    the job's queue, and traces one bounce.
-//  * Requests arrive through a submission ring in pinned host memory: the
-//    host writes an entry and publishes its sequence word; the last block
-//    to reach each BFS barrier reads new entries (system-scope loads) and
-//    installs them — BFS queries into a slot whose depth array the sub-grid
-//    clears during the next level and whose source enters the frontier for
-//    the level after; tile jobs into an RT job slot that the RT waves pick
-//    up at once.  Completions go back through a pinned completion ring.
+//  * Requests reach the kernel through two device-resident submission
+//    queues in HBM, one for BFS queries and one for tile jobs: G-LFQ rings
+//    of slot numbers beside a per-slot request descriptor.  The host's only
+//    channel in is a doorbell ring in pinned memory (an entry per request,
+//    its sequence word written last).  One gate wave, the first wave after
+//    the BFS sub-grid, reads new doorbell entries (system-scope loads), copies
+//    each request into its slot's descriptor and pushes the slot onto its
+//    queue; nothing else in the kernel reads host memory on the way in.
+//  * The persistent consumers pull from those queues: the last block to
+//    reach each BFS barrier pops BFS queries and installs them into a slot
+//    whose depth array the sub-grid clears during the next level and whose
+//    source enters the frontier for the level after; an RT wave that finds
+//    no work, and every RT wave once in kRtSubmitEvery passes, pops tile
+//    jobs and installs them into an RT job slot at once, so a tile never
+//    waits for the BFS level cadence.  Completions go back through a pinned
+//    completion ring.
 //  * The host admission loop decides when a request may enter: a free slot,
 //    room in the device-memory budget (depth arrays, ray records, images)
 //    and in the frontier ring (the sum of V of in-flight queries).
@@ -46,11 +55,13 @@ namespace sv {
 
 using namespace gcqf;
 
-constexpr uint32_t kMaxAdmit = 32;           // entries installed per BFS barrier
+constexpr uint32_t kMaxAdmit = 32;           // BFS queries installed per BFS barrier
 constexpr uint64_t kRtIdleLimit = 1ull << 30;  // idle polls of an RT wave before it gives up
 constexpr uint32_t kMaxTilePixels = 1u << 16;  // RT items carry a 16-bit pixel and a 15-bit job generation
+constexpr uint32_t kRtSubmitEvery = 32;        // passes of a busy RT wave between looks at the tile queue
 
-// Submission ring entry (pinned host memory, written by the host; seq last).
+// Doorbell entry (pinned host memory, written by the host; seq last).  The
+// gate wave copies it into its slot's descriptor in HBM (the same layout).
 struct alignas(64) Req {
   uint64_t seq;  // i + 1 once entry i is complete
   uint64_t id;   // server-internal request number
@@ -90,10 +101,12 @@ static_assert(sizeof(RtJob) == 512, "rt job");
 
 struct Ctl {
   Barrier bar;
-  uint32_t err, p1[31];  // 1 frontier enqueue, 2 RT enqueue, 4 completion ring full, 8 RT idle limit
+  uint32_t err, p1[31];  // 1 frontier enqueue, 2 RT enqueue, 4 completion ring full, 8 RT idle limit,
+                         // 16 submission queue full
   uint32_t exit, p2[31];
   uint32_t adm_n, adm_slot[kMaxAdmit], p3[31];
-  uint64_t sub_head, p4[15];
+  uint64_t sub_head, p4[15];  // doorbell entries the gate wave has moved
+  uint32_t stop_seen, p9[31];  // the gate wave saw the host's stop word
   uint64_t comp_tail, p5[15];
   int32_t active_bfs, p6[31];
   int32_t active_rt, p7[31];
@@ -108,13 +121,15 @@ struct Dev {
   RtJob* jobs;
   char* rings;             // one G-LFQ ring per RT job slot
   uint64_t ring_stride;
-  Req* sub;
+  Req* sub;                // doorbell ring (pinned host memory)
+  gcq::lfq::Ring* subq[2]; // device-resident submission queues: BFS slots, RT job slots
+  Req* desc[2];            // per-slot request descriptors in HBM: BFS, RT
   Comp* comp;
   const uint32_t* stop;    // set by the host once every request has completed
   Ctl* c;
   rtk::Scene sc;
   int32_t W, H, max_depth;
-  uint32_t njobs, bfs_blocks, max_requests;
+  uint32_t njobs, bfs_blocks, max_requests, rt_workers;  // rt_workers: the RT sub-grid runs tile jobs
 };
 
 __device__ void publish(const Dev& d, uint64_t id, int32_t type, int32_t levels, int32_t slot, uint64_t rays) {
@@ -134,61 +149,109 @@ __device__ void publish(const Dev& d, uint64_t id, int32_t type, int32_t levels,
   gcq::st_sys(&e->seq, k + 1);
 }
 
-// Install new requests (one thread: the last block to reach the barrier
-// that ends level L).
-__device__ void admit(const Dev& d, int32_t L) {
+// The gate wave: move each complete doorbell entry into its slot's
+// descriptor and push the slot onto the BFS or RT submission queue, until
+// the kernel exits.  Lane 0 does the work.  The host writes an entry only
+// for a slot it has freed, so a slot is in at most one queue at a time and
+// each queue holds at most its slot count (its capacity).
+__device__ void gate(const Dev& d) {
   Ctl* c = d.c;
-  uint64_t h = gcq::ld_rlx(&c->sub_head);
-  uint32_t n = 0;
-  while (n < kMaxAdmit && h < d.max_requests) {
-    Req* e = &d.sub[h];
-    if (gcq::ld_sys(&e->seq) != h + 1) break;
-    gcq::vm_wait();  // the sequence word before the fields
-    const int32_t type = gcq::ld_sys(&e->type), slot = gcq::ld_sys(&e->slot);
-    const uint64_t id = gcq::ld_sys(&e->id), buf = gcq::ld_sys(&e->buf);
-    if (type == GCQ_REQ_BFS) {
-      const GraphDev& g = d.graphs[gcq::ld_sys(&e->graph)];
-      bk::Slot* s = &d.slots[slot];
-      gcq::st_rlx(&s->off, g.off);
-      gcq::st_rlx(&s->idx, g.idx);
-      gcq::st_rlx(&s->depth, buf);
-      gcq::st_rlx(&s->visited, gcq::ld_sys(&e->buf2));
-      gcq::st_rlx(&s->src, (uint32_t)gcq::ld_sys(&e->src));
-      gcq::st_rlx(&s->nv, g.nv);
-      gcq::st_rlx(&s->start, (uint32_t)(L + 2));  // cleared during L+1, source expanded at L+2
-      gcq::st_rlx(&s->pad2[0], id);
-      gcq::st_rlx(&d.bd.alive[slot * bk::kAliveStride], (uint32_t)(L + 2));
-      gcq::st_rlx(&d.slot_active[slot], 1u);
-      gcq::st_rlx(&c->adm_slot[n], (uint32_t)slot);
-      ++n;
-      gcq::faa_rlx(&c->active_bfs, 1);
-    } else {
-      RtJob* j = &d.jobs[slot];
-      const uint32_t gen = (gcq::ld_rlx(&j->gen) + 1u) & 0x7FFFu;
-      const int32_t w = gcq::ld_sys(&e->w), hh = gcq::ld_sys(&e->h);
-      gcq::st_rlx(&j->img, buf);
-      gcq::st_rlx(&j->recs, gcq::ld_sys(&e->buf2));
-      gcq::st_rlx(&j->id, id);
-      gcq::st_rlx(&j->x0, gcq::ld_sys(&e->x0));
-      gcq::st_rlx(&j->y0, gcq::ld_sys(&e->y0));
-      gcq::st_rlx(&j->w, w);
-      gcq::st_rlx(&j->h, hh);
-      gcq::st_rlx(&j->npx, (uint32_t)(w * hh));
-      gcq::st_rlx(&j->gen, gen);
-      gcq::st_rlx(&j->alive, (uint64_t)(w * hh));
-      gcq::st_rlx(&j->traced, (uint64_t)0);
-      gcq::vm_wait();  // fields before the claim counter that carries the generation
-      gcq::st_rlx(&j->next_primary, (uint64_t)gen << 32);
-      gcq::vm_wait();
-      gcq::st_rlx(&j->active, 1u);
-      gcq::faa_rlx(&c->active_rt, 1);
+  uint64_t h = 0;
+  for (;;) {
+    uint32_t moved = 0, quit = 0;
+    if (gcq::lane_id() == 0) {
+      while (h < d.max_requests) {
+        const Req* e = &d.sub[h];
+        if (gcq::ld_sys(&e->seq) != h + 1) break;
+        gcq::vm_wait();  // the sequence word before the fields
+        const int32_t type = gcq::ld_sys(&e->type), slot = gcq::ld_sys(&e->slot);
+        const int q = type == GCQ_REQ_BFS ? 0 : 1;
+        Req* dst = &d.desc[q][slot];
+        gcq::st_rlx(&dst->id, gcq::ld_sys(&e->id));
+        gcq::st_rlx(&dst->type, type);
+        gcq::st_rlx(&dst->graph, gcq::ld_sys(&e->graph));
+        gcq::st_rlx(&dst->src, gcq::ld_sys(&e->src));
+        gcq::st_rlx(&dst->slot, slot);
+        gcq::st_rlx(&dst->x0, gcq::ld_sys(&e->x0));
+        gcq::st_rlx(&dst->y0, gcq::ld_sys(&e->y0));
+        gcq::st_rlx(&dst->w, gcq::ld_sys(&e->w));
+        gcq::st_rlx(&dst->h, gcq::ld_sys(&e->h));
+        gcq::st_rlx(&dst->buf, gcq::ld_sys(&e->buf));
+        gcq::st_rlx(&dst->buf2, gcq::ld_sys(&e->buf2));
+        // enqueue waits for these stores before it publishes the slot
+        gcq::lfq::enqueue(d.subq[q], (uint32_t)slot);
+        ++h;
+        ++moved;
+      }
+      if (moved) gcq::st_rlx(&c->sub_head, h);
+      else if (gcq::ld_sys(d.stop)) gcq::st_rlx(&c->stop_seen, 1u);
+      quit = gcq::ld_rlx(&c->exit) | gcq::ld_rlx(&c->bar.abort);
     }
-    ++h;
+    if (gcq::bcast32(quit, 0)) break;
+    if (!gcq::bcast32(moved, 0)) __builtin_amdgcn_s_sleep(8);
   }
-  gcq::st_rlx(&c->sub_head, h);
+}
+
+// Install queued BFS queries (one thread: the last block to reach the
+// barrier that ends level L).
+__device__ void admit_bfs(const Dev& d, int32_t L) {
+  Ctl* c = d.c;
+  uint32_t n = 0;
+  while (n < kMaxAdmit) {
+    const uint32_t slot = gcq::lfq::dequeue(d.subq[0], (gcq::lfq::NoProbe*)nullptr, true);
+    if (slot == gcq::lfq::kBot) break;
+    const Req* e = &d.desc[0][slot];
+    const GraphDev& g = d.graphs[gcq::ld_rlx(&e->graph)];
+    bk::Slot* s = &d.slots[slot];
+    gcq::st_rlx(&s->off, g.off);
+    gcq::st_rlx(&s->idx, g.idx);
+    gcq::st_rlx(&s->depth, gcq::ld_rlx(&e->buf));
+    gcq::st_rlx(&s->visited, gcq::ld_rlx(&e->buf2));
+    gcq::st_rlx(&s->src, (uint32_t)gcq::ld_rlx(&e->src));
+    gcq::st_rlx(&s->nv, g.nv);
+    gcq::st_rlx(&s->start, (uint32_t)(L + 2));  // cleared during L+1, source expanded at L+2
+    gcq::st_rlx(&s->pad2[0], gcq::ld_rlx(&e->id));
+    gcq::st_rlx(&d.bd.alive[slot * bk::kAliveStride], (uint32_t)(L + 2));
+    gcq::st_rlx(&d.slot_active[slot], 1u);
+    gcq::st_rlx(&c->adm_slot[n], slot);
+    ++n;
+    gcq::faa_rlx(&c->active_bfs, 1);
+  }
   gcq::st_rlx(&c->adm_n, n);
-  if (n == 0 && gcq::ld_sys(d.stop) && gcq::ld_rlx(&c->active_bfs) == 0 && gcq::ld_rlx(&c->active_rt) == 0)
+  // The host sets its stop word only once every request has completed, and
+  // the gate wave reports it only after finding no doorbell entry left.
+  if (n == 0 && gcq::ld_rlx(&c->stop_seen) && gcq::ld_rlx(&c->active_bfs) == 0 &&
+      gcq::ld_rlx(&c->active_rt) == 0)
     gcq::st_rlx(&c->exit, 1u);
+}
+
+// Install one queued tile job, if any (lane 0 of an RT wave); true if one was.  The job's
+// fields land before the claim counter that carries its generation, and
+// that before the slot reads as active.
+__device__ bool admit_rt(const Dev& d) {
+  const uint32_t slot = gcq::lfq::dequeue(d.subq[1], (gcq::lfq::NoProbe*)nullptr, true);
+  if (slot == gcq::lfq::kBot) return false;
+  const Req* e = &d.desc[1][slot];
+  RtJob* j = &d.jobs[slot];
+  const uint32_t gen = (gcq::ld_rlx(&j->gen) + 1u) & 0x7FFFu;
+  const int32_t w = gcq::ld_rlx(&e->w), hh = gcq::ld_rlx(&e->h);
+  gcq::st_rlx(&j->img, gcq::ld_rlx(&e->buf));
+  gcq::st_rlx(&j->recs, gcq::ld_rlx(&e->buf2));
+  gcq::st_rlx(&j->id, gcq::ld_rlx(&e->id));
+  gcq::st_rlx(&j->x0, gcq::ld_rlx(&e->x0));
+  gcq::st_rlx(&j->y0, gcq::ld_rlx(&e->y0));
+  gcq::st_rlx(&j->w, w);
+  gcq::st_rlx(&j->h, hh);
+  gcq::st_rlx(&j->npx, (uint32_t)(w * hh));
+  gcq::st_rlx(&j->gen, gen);
+  gcq::st_rlx(&j->alive, (uint64_t)(w * hh));
+  gcq::st_rlx(&j->traced, (uint64_t)0);
+  gcq::vm_wait();
+  gcq::st_rlx(&j->next_primary, (uint64_t)gen << 32);
+  gcq::vm_wait();
+  gcq::faa_rlx(&d.c->active_rt, 1);  // counted before the job can complete
+  gcq::st_rlx(&j->active, 1u);
+  return true;
 }
 
 // Block 0 after the barrier ending level L: queries with nothing discovered
@@ -236,7 +299,7 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) 
     (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage);
     if (!grid_barrier(&d.c->bar, d.bfs_blocks, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
-          admit(d, L);
+          admit_bfs(d, L);
         }))
       break;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -252,6 +315,7 @@ __device__ void rt_part(const Dev& d) {
   const rtk::V3 light = rtk::norm(d.sc.light_raw);
   uint32_t j = (rtid / 64) % d.njobs;
   uint64_t idle = 0;
+  uint32_t pass = rtid / 64;  // staggers the busy waves' looks at the tile queue
   for (;;) {
     int32_t pick = -1;
     for (uint32_t k = 0; k < d.njobs; ++k) {
@@ -260,6 +324,11 @@ __device__ void rt_part(const Dev& d) {
         pick = (int32_t)jj;
         break;
       }
+    }
+    if (pick < 0 || ++pass % kRtSubmitEvery == 0) {
+      uint32_t got = 0;
+      if (lane == 0) got = admit_rt(d) ? 1u : 0u;
+      if (gcq::bcast32(got, 0) && pick < 0) continue;  // pick the job just installed
     }
     if (pick < 0) {
       if (gcq::ld_rlx(&d.c->exit)) break;
@@ -349,11 +418,14 @@ __device__ void rt_part(const Dev& d) {
   }
 }
 
+// Blocks [0, bfs_blocks) are the BFS sub-grid; the first wave after it is
+// the gate; the waves after that run tile jobs (when the server has a scene).
 template <class F>
 __global__ void k_serve(Frontiers<F> fq, Dev d) {
   __shared__ uint32_t stage[16][128];
   if (blockIdx.x < d.bfs_blocks) bfs_part(fq, d, stage[(threadIdx.x / 64) & 15]);
-  else rt_part(d);
+  else if (blockIdx.x == d.bfs_blocks && threadIdx.x < 64) gate(d);
+  else if (d.rt_workers) rt_part(d);
 }
 
 __global__ void k_init_rings(char* pool, uint64_t stride, uint32_t n, uint32_t nrings) {
@@ -434,6 +506,9 @@ struct gcq_server {
   char* rings = nullptr;
   uint64_t ring_stride = 0;
   sv::Ctl* ctl = nullptr;
+  char* subq = nullptr;        // the two submission queues (G-LFQ rings): BFS, RT
+  uint64_t subq_off = 0;       // offset of the RT queue
+  sv::Req* desc = nullptr;     // request descriptors: bfs_slots BFS, then rt_slots RT
   char* arena_base = nullptr;  // in-flight request state, budget_bytes
   sv::Arena arena;
   char* keep_base = nullptr;   // kept results, keep_bytes
@@ -464,7 +539,8 @@ constexpr double kStopTimeoutMs = 30000.0;
 
 void release_all(gcq_server* s) {
   for (void* p : {(void*)s->slots, (void*)s->alive, (void*)s->slot_active, (void*)s->dgraphs, (void*)s->jobs,
-                  (void*)s->rings, (void*)s->ctl, (void*)s->arena_base, (void*)s->keep_base})
+                  (void*)s->rings, (void*)s->ctl, (void*)s->subq, (void*)s->desc, (void*)s->arena_base,
+                  (void*)s->keep_base})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)s->sub, (void*)s->comp, (void*)s->stop})
     if (p) (void)hipHostFree(p);
@@ -632,7 +708,7 @@ extern "C" int gcq_server_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_
   GCQ_REQUIRE(cfg->block >= 64 && cfg->block <= 1024 && cfg->block % 64 == 0, "bad block");
   GCQ_REQUIRE(cfg->bfs_threads >= cfg->block && cfg->bfs_threads % cfg->block == 0, "bad bfs_threads");
   GCQ_REQUIRE(cfg->rt_threads % cfg->block == 0, "bad rt_threads");
-  GCQ_REQUIRE(!scene || cfg->rt_threads > 0, "a scene needs rt_threads > 0");
+  GCQ_REQUIRE(!scene || cfg->rt_threads >= 128, "a scene needs rt_threads >= 128 (the gate wave takes one)");
   GCQ_REQUIRE(cfg->bfs_slots >= 1 && cfg->rt_slots >= 1 && cfg->rt_slots <= 1024, "bad slot counts");
   GCQ_REQUIRE(cfg->frontier_log2 >= 10 && cfg->frontier_log2 <= 30, "frontier_log2 must be in [10, 30]");
   GCQ_REQUIRE(cfg->pop_width >= 1 && cfg->pop_width <= 64 && cfg->flush_at >= 1 && cfg->flush_at <= 64,
@@ -690,6 +766,12 @@ extern "C" int gcq_server_start(gcq_server* s) {
   s->ring_stride = (gcq::lfq::ring_bytes(sv::kMaxTilePixels) + 255) & ~255ull;
   GCQ_HIP(hipMalloc(&s->rings, s->ring_stride * J));
   GCQ_HIP(hipMalloc(&s->ctl, sizeof(sv::Ctl)));
+  // Submission queues: a slot is queued at most once, so capacity = slots.
+  const uint32_t qn[2] = {gcqf::pow2_at_least(std::max<uint64_t>(S, 64)),
+                          gcqf::pow2_at_least(std::max<uint64_t>(J, 64))};
+  s->subq_off = (gcq::lfq::ring_bytes(qn[0]) + 255) & ~255ull;
+  GCQ_HIP(hipMalloc(&s->subq, s->subq_off + gcq::lfq::ring_bytes(qn[1])));
+  GCQ_HIP(hipMalloc(&s->desc, sizeof(sv::Req) * (S + J)));
   GCQ_HIP(hipMalloc(&s->arena_base, c.budget_bytes));
   s->arena.init(c.budget_bytes);
   if (c.keep_bytes) {
@@ -719,6 +801,10 @@ extern "C" int gcq_server_start(gcq_server* s) {
   s->fs.reset(s->stream);
   hipLaunchKernelGGL(sv::k_init_rings, dim3(gcqf::grid_of(2ull * sv::kMaxTilePixels)), dim3(256), 0, s->stream,
                      s->rings, s->ring_stride, sv::kMaxTilePixels, J);
+  for (int q = 0; q < 2; ++q)
+    hipLaunchKernelGGL(sv::k_init_rings, dim3(gcqf::grid_of(2ull * qn[q])), dim3(256), 0, s->stream,
+                       s->subq + (q ? s->subq_off : 0), (uint64_t)0, qn[q], 1u);
+  zero(s->desc, sizeof(sv::Req) * (S + J));
   GCQ_HIP(hipGetLastError());
   GCQ_HIP(hipStreamSynchronize(s->stream));
 
@@ -731,6 +817,10 @@ extern "C" int gcq_server_start(gcq_server* s) {
   d.rings = s->rings;
   d.ring_stride = s->ring_stride;
   d.sub = s->sub;
+  d.subq[0] = (gcq::lfq::Ring*)s->subq;
+  d.subq[1] = (gcq::lfq::Ring*)(s->subq + s->subq_off);
+  d.desc[0] = s->desc;
+  d.desc[1] = s->desc + S;
   d.comp = s->comp;
   d.stop = s->stop;
   d.c = s->ctl;
@@ -741,7 +831,10 @@ extern "C" int gcq_server_start(gcq_server* s) {
   d.njobs = J;
   d.bfs_blocks = c.bfs_threads / c.block;
   d.max_requests = c.max_requests;
-  const uint32_t grid = (c.bfs_threads + c.rt_threads) / c.block;
+  d.rt_workers = s->scene ? 1u : 0u;
+  // The gate wave leads the first block after the BFS sub-grid; a server
+  // without RT threads gets one block for it.
+  const uint32_t grid = (c.bfs_threads + c.rt_threads) / c.block + (c.rt_threads == 0 ? 1u : 0u);
   int dev = 0, cus = 0, per_cu = 0;
   GCQ_HIP(hipGetDevice(&dev));
   GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

@@ -218,10 +218,14 @@ int gcq_graph_destroy(gcq_graph* g);
 enum gcq_bfs_mode {
   GCQ_BFS_HOST_LOOP = 0,  /* one launch + host sync per level (bfs_bench.cpp:583-623) */
   GCQ_BFS_PERSISTENT = 1, /* one launch, device-side level barrier */
-  GCQ_BFS_ASYNC = 2       /* one launch, no level barrier: one frontier queue, a vertex whose depth falls
+  GCQ_BFS_ASYNC = 2,      /* one launch, no level barrier: one frontier queue, a vertex whose depth falls
                              (unsigned atomic min) is pushed again, the run ends when no pushed item is left
                              unexpanded.  Same depths bit for bit; edges_scanned counts every expansion,
                              repeated ones included.  Ring frontiers only (G-WFQ, G-LFQ, SFQ). */
+  GCQ_BFS_BITMAP = 3      /* as PERSISTENT, but a vertex is claimed by fetch-or in a visited bitmap (one
+                             bit per vertex, probed through the caches first) and its depth written by a
+                             plain store, instead of a CAS on its depth word: visited neighbours cost a
+                             cached load rather than an atomic.  Same depths bit for bit. */
 };
 typedef struct gcq_bfs_config {
   int32_t kind;       /* frontier queue: enum gcq_kind */
@@ -399,10 +403,13 @@ int gcq_rt_append_csv(const char* path, int32_t style, const gcq_rt_csv_row* row
  * admitted by a host loop as slots and device memory free up and run in one
  * persistent kernel (a BFS sub-grid advancing every admitted query level by
  * level, an RT sub-grid tracing admitted tiles).  Requests reach the kernel
- * through a submission ring in pinned host memory that it polls at every
- * level barrier; completions come back through a pinned completion ring.
- * While the server runs nothing else is launched, so no stream can queue
- * behind the persistent kernel. */
+ * through two device-resident submission queues (G-LFQ rings in HBM, one
+ * for BFS queries and one for tile jobs), fed by one gate wave of the kernel
+ * from a doorbell ring in pinned host memory; the BFS sub-grid takes queries
+ * at its level barriers, RT waves take tile jobs as soon as they look for
+ * work.  Completions come back through a pinned completion ring.  While the
+ * server runs nothing else is launched, so no stream can queue behind the
+ * persistent kernel. */
 typedef struct gcq_server gcq_server;
 enum gcq_req_type { GCQ_REQ_BFS = 0, GCQ_REQ_RT = 1 };
 typedef struct gcq_request {
@@ -425,7 +432,8 @@ typedef struct gcq_completion {
 typedef struct gcq_server_config {
   int32_t kind;            /* BFS frontier queue (enum gcq_kind) */
   uint32_t bfs_threads;    /* BFS sub-grid, multiple of block */
-  uint32_t rt_threads;     /* RT sub-grid, multiple of block */
+  uint32_t rt_threads;     /* RT sub-grid, multiple of block; its first wave is the gate (>= 128 with a
+                              scene; 0 without one adds a block for the gate) */
   uint32_t block;
   uint32_t bfs_slots;      /* concurrent BFS queries */
   uint32_t rt_slots;       /* concurrent RT tile jobs */

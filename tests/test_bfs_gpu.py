@@ -8,7 +8,7 @@ pytestmark = pytest.mark.gpu
 
 KINDS = [0, 1, 2, 3]
 NAMES = {0: "gwfq", 1: "glfq", 2: "wfq", 3: "sfq"}
-MODES = {0: "hostloop", 1: "persistent", 2: "async"}
+MODES = {0: "hostloop", 1: "persistent", 2: "async", 3: "bitmap"}
 RING_KINDS = [0, 1, 3]  # the barrier-free mode takes ring frontiers
 
 
@@ -39,7 +39,7 @@ def _run_and_check(orc, g, src, kind, threads, mode, off=None, idx=None):
     return res
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2], ids=[MODES[m] for m in (0, 1, 2)])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3], ids=[MODES[m] for m in (0, 1, 2, 3)])
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
 def test_lattice_small(gcq, orc, kind, mode):
     if mode == 2 and kind == 2:
@@ -69,7 +69,7 @@ def test_special_graphs(gcq, orc, kind):
     for name, ((off, idx), src) in cases.items():
         print(f"[bfs special] {name}", flush=True)
         with bfs.Graph.from_csr(off, idx) as g:
-            for mode in ((0, 1, 2) if kind in RING_KINDS else (0, 1)):
+            for mode in ((0, 1, 2, 3) if kind in RING_KINDS else (0, 1, 3)):
                 _run_and_check(orc, g, src, kind, 1024, mode, off, idx)
 
 
@@ -83,6 +83,7 @@ def test_rmat_hubs(gcq, orc, kind):
         for k in range(2):
             _run_and_check(orc, g, g.pick_source(11, k), kind, 8192, 1, off, idx)
         _run_and_check(orc, g, g.pick_source(11, 2), kind, 8192, 0, off, idx)
+        _run_and_check(orc, g, g.pick_source(11, 4), kind, 8192, 3, off, idx)
         if kind in RING_KINDS:
             _run_and_check(orc, g, g.pick_source(11, 3), kind, 8192, 2, off, idx)
 
@@ -110,12 +111,12 @@ def test_wfq_cells_recycled_65536_threads(gcq, orc, mode):
 def test_config3_lattice_4900(gcq, orc, kind):
     # The benchmark graph: 4900 x 4900 lattice, p = 0.75, 1% diagonals,
     # seed 7, source 0, 8,192 threads (block 256): persistent (9,371 levels
-    # of counted pops, each level's Head set back at its barrier) and, for the
-    # ring frontiers, barrier-free.
+    # of counted pops, each level's Head set back at its barrier), the same
+    # with visited-bitmap claims and, for the ring frontiers, barrier-free.
     from gcq_amd import bfs
     with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as g:
         off, idx = g.csr()
-        for mode in ((1, 2) if kind in RING_KINDS else (1,)):
+        for mode in ((1, 3, 2) if kind in RING_KINDS else (1, 3)):
             res = _run_and_check(orc, g, 0, kind, 8192, mode, off, idx)
             print(f"[bfs cfg3] mode={MODES[mode]} levels={res['levels']} reached={res['reached']} "
                   f"kernel_ms={res['kernel_ms']:.1f} scanned={res['edges_scanned']}", flush=True)
