@@ -287,7 +287,7 @@ def profile_run(a):
         with rt.Scene.preset(rt.RANDOM, 64, 3) as scene:
             for _ in range(a.runs):
                 with serve.Server(graphs, scene, (1920, 1080), bfs_threads=229376, block=512, rt_threads=32768,
-                                  frontier_log2=28) as srv:
+                                  frontier_log2=28, small_log2=18, small_threads=32768) as srv:
                     res = serve.latency_summary(srv.run_trace(rows))
         out.update(res)
         for gr in graphs:
@@ -434,11 +434,22 @@ def secondary(g, with_cpu):
     lam = 150.0
     rows = serve.poisson_trace(300, lam, nvs, frame, rt_fraction=0.1, seed=5, graph_sources=pick)
     burst = serve.poisson_trace(300, 1e9, nvs, frame, rt_fraction=0.1, seed=6, graph_sources=pick)
-    scfg = dict(bfs_threads=229376, block=512, rt_threads=32768, frontier_log2=28)
+    # two BFS lanes: graphs of at most 2^18 vertices (scales 14..18) get
+    # 32,768 of the BFS threads and their own level barrier
+    scfg = dict(bfs_threads=229376, block=512, rt_threads=32768, frontier_log2=28, small_log2=18,
+                small_threads=32768)
     with serve.Server(graphs, scene, frame, **scfg) as srv:
-        s_trace = serve.latency_summary(srv.run_trace(rows))
+        comps = srv.run_trace(rows)
+        s_trace = serve.latency_summary(comps)
+        by_lane = {lane: serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 0 and
+                                               (nvs[r["graph"]] <= 1 << 18) == (lane == "small")])
+                   for lane in ("small", "big")}
+        by_lane["tiles"] = serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 1])
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         s_burst = serve.latency_summary(srv.run_trace(burst))
+    one = {k: v for k, v in scfg.items() if k not in ("small_log2", "small_threads")}
+    with serve.Server(graphs, scene, frame, **one) as srv:
+        s_one = serve.latency_summary(srv.run_trace(rows))
     pcfg = dict(bfs_threads=65536, rt_threads_per_tile=8192)
     p_trace = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows, **pcfg))
     p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, **pcfg))
@@ -447,6 +458,8 @@ def secondary(g, with_cpu):
         "workload": f"Poisson {lam:.0f} req/s, 300 requests: 90% BFS on R-MAT scales 14..22, 10% 256x256 tiles "
                     "of the 1920x1080 64-sphere scene (4 bounces), 64 GB budget, arrival seed 5",
         "served": rnd(s_trace), "plain": rnd(p_trace),
+        "served_by_class": {k: rnd(v) for k, v in by_lane.items()},
+        "served_one_lane": rnd(s_one),
         "burst_served": rnd(s_burst), "burst_plain": rnd(p_burst),
         "capacity_vs_plain": round(s_burst["throughput_rps"] / p_burst["throughput_rps"], 2),
         "config": dict(scfg, plain_bfs_threads=pcfg["bfs_threads"],

@@ -9,6 +9,7 @@
 // Capacity is sized by the host to the largest frontier, so pushes do not
 // fail (the reference ignores failed pushes, bfs_bench.cpp:380, 404, 475).
 #pragma once
+#include <type_traits>
 #include <algorithm>
 
 #include "queue_impl.hpp"
@@ -460,30 +461,27 @@ struct FrontierSet {
     }
   }
 
+  // This set's queues through adapter F (which must match its kind).
+  template <class F>
+  Frontiers<F> view() const {
+    Frontiers<F> f;
+    for (int i = 0; i < 3; ++i) {
+      if constexpr (std::is_same_v<F, FGwf>) f.q[i] = FGwf{(gcq::gwf::Ring*)q[i % 2]};
+      else if constexpr (std::is_same_v<F, FLfq>) f.q[i] = FLfq{(gcq::lfq::Ring*)q[i % 2]};
+      else if constexpr (std::is_same_v<F, FSfq>) f.q[i] = FSfq{(gcq::sfq::Queue*)q[i % 2]};
+      else f.q[i] = FYmc{(gcq::ymc::Queue*)q[i], (gcq::ymc::Handle*)h[i]};
+    }
+    return f;
+  }
+
   // Call fn(Frontiers<F>) with the adapter type of this set's kind.
   template <class Fn>
   int dispatch(Fn&& fn) const {
     switch (kind) {
-      case GCQ_GWFQ: {
-        Frontiers<FGwf> f;
-        for (int i = 0; i < 3; ++i) f.q[i] = FGwf{(gcq::gwf::Ring*)q[i % 2]};
-        return fn(f);
-      }
-      case GCQ_GLFQ: {
-        Frontiers<FLfq> f;
-        for (int i = 0; i < 3; ++i) f.q[i] = FLfq{(gcq::lfq::Ring*)q[i % 2]};
-        return fn(f);
-      }
-      case GCQ_SFQ: {
-        Frontiers<FSfq> f;
-        for (int i = 0; i < 3; ++i) f.q[i] = FSfq{(gcq::sfq::Queue*)q[i % 2]};
-        return fn(f);
-      }
-      case GCQ_WFQ: {
-        Frontiers<FYmc> f;
-        for (int i = 0; i < 3; ++i) f.q[i] = FYmc{(gcq::ymc::Queue*)q[i], (gcq::ymc::Handle*)h[i]};
-        return fn(f);
-      }
+      case GCQ_GWFQ: return fn(view<FGwf>());
+      case GCQ_GLFQ: return fn(view<FLfq>());
+      case GCQ_SFQ: return fn(view<FSfq>());
+      case GCQ_WFQ: return fn(view<FYmc>());
     }
     GCQ_REQUIRE(false, "frontier set not allocated");
   }

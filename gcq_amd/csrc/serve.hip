@@ -83,7 +83,7 @@ static_assert(sizeof(Comp) == 64, "comp");
 
 struct GraphDev {
   uint64_t off, idx;
-  uint32_t nv, pad;
+  uint32_t nv, lane;  // lane: the BFS lane that serves this graph's queries
   uint64_t pad2;
 };
 
@@ -99,30 +99,42 @@ struct RtJob {
 };
 static_assert(sizeof(RtJob) == 512, "rt job");
 
-struct Ctl {
+// One BFS lane: a sub-grid with its own level barrier, frontier queues and
+// submission queue.  The second lane, when configured, takes the queries on
+// graphs of at most 2^small_log2 vertices, so that they advance at their
+// own level cadence instead of waiting out the big graphs' levels.
+constexpr int kLanes = 2;
+struct LaneCtl {
   Barrier bar;
-  uint32_t err, p1[31];  // 1 frontier enqueue, 2 RT enqueue, 4 completion ring full, 8 RT idle limit,
-                         // 16 submission queue full
-  uint32_t exit, p2[31];
   uint32_t adm_n, adm_slot[kMaxAdmit], p3[31];
+  int32_t active, p6[31];
+  int32_t levels, p8[31];
+  // Set only by the lane's own last block inside its barrier, so every block
+  // of the lane reads the same value after that barrier (the shared exit
+  // word can be set by the other lane at any time).
+  uint32_t quit, p9[31];
+};
+
+struct Ctl {
+  LaneCtl lane[kLanes];
+  uint32_t err, p1[31];  // 1 frontier enqueue, 2 RT enqueue, 4 completion ring full, 8 RT idle limit
+  uint32_t exit, p2[31];
   uint64_t sub_head, p4[15];  // doorbell entries the gate wave has moved
   uint32_t stop_seen, p9[31];  // the gate wave saw the host's stop word
   uint64_t comp_tail, p5[15];
-  int32_t active_bfs, p6[31];
   int32_t active_rt, p7[31];
-  int32_t levels, p8[31];
 };
 
 struct Dev {
   bk::Dev bd;              // read side of the BFS slot table
   bk::Slot* slots;         // the same table, written at admission
-  uint32_t* slot_active;   // per BFS slot
+  uint32_t* slot_active;   // per BFS slot: 1 + the lane serving it, 0 when free
   const GraphDev* graphs;
   RtJob* jobs;
   char* rings;             // one G-LFQ ring per RT job slot
   uint64_t ring_stride;
   Req* sub;                // doorbell ring (pinned host memory)
-  gcq::lfq::Ring* subq[2]; // device-resident submission queues: BFS slots, RT job slots
+  gcq::lfq::Ring* subq[3]; // device-resident submission queues: BFS slots (lane 0, lane 1), RT job slots
   Req* desc[2];            // per-slot request descriptors in HBM: BFS, RT
   Comp* comp;
   const uint32_t* stop;    // set by the host once every request has completed
@@ -130,6 +142,7 @@ struct Dev {
   rtk::Scene sc;
   int32_t W, H, max_depth;
   uint32_t njobs, bfs_blocks, max_requests, rt_workers;  // rt_workers: the RT sub-grid runs tile jobs
+  uint32_t lane_blocks[kLanes];  // blocks [0, lane_blocks[0]) are lane 0, the next lane_blocks[1] lane 1
 };
 
 __device__ void publish(const Dev& d, uint64_t id, int32_t type, int32_t levels, int32_t slot, uint64_t rays) {
@@ -164,12 +177,13 @@ __device__ void gate(const Dev& d) {
         const Req* e = &d.sub[h];
         if (gcq::ld_sys(&e->seq) != h + 1) break;
         gcq::vm_wait();  // the sequence word before the fields
-        const int32_t type = gcq::ld_sys(&e->type), slot = gcq::ld_sys(&e->slot);
-        const int q = type == GCQ_REQ_BFS ? 0 : 1;
-        Req* dst = &d.desc[q][slot];
+        const int32_t type = gcq::ld_sys(&e->type), slot = gcq::ld_sys(&e->slot), graph = gcq::ld_sys(&e->graph);
+        const int t = type == GCQ_REQ_BFS ? 0 : 1;
+        const int q = t == 0 ? (int)d.graphs[graph].lane : kLanes;
+        Req* dst = &d.desc[t][slot];
         gcq::st_rlx(&dst->id, gcq::ld_sys(&e->id));
         gcq::st_rlx(&dst->type, type);
-        gcq::st_rlx(&dst->graph, gcq::ld_sys(&e->graph));
+        gcq::st_rlx(&dst->graph, graph);
         gcq::st_rlx(&dst->src, gcq::ld_sys(&e->src));
         gcq::st_rlx(&dst->slot, slot);
         gcq::st_rlx(&dst->x0, gcq::ld_sys(&e->x0));
@@ -185,20 +199,21 @@ __device__ void gate(const Dev& d) {
       }
       if (moved) gcq::st_rlx(&c->sub_head, h);
       else if (gcq::ld_sys(d.stop)) gcq::st_rlx(&c->stop_seen, 1u);
-      quit = gcq::ld_rlx(&c->exit) | gcq::ld_rlx(&c->bar.abort);
+      quit = gcq::ld_rlx(&c->exit) | gcq::ld_rlx(&c->lane[0].bar.abort) | gcq::ld_rlx(&c->lane[1].bar.abort);
     }
     if (gcq::bcast32(quit, 0)) break;
     if (!gcq::bcast32(moved, 0)) __builtin_amdgcn_s_sleep(8);
   }
 }
 
-// Install queued BFS queries (one thread: the last block to reach the
-// barrier that ends level L).
-__device__ void admit_bfs(const Dev& d, int32_t L) {
+// Install a lane's queued BFS queries (one thread: the last block to reach
+// the barrier that ends the lane's level L).
+__device__ void admit_bfs(const Dev& d, int lane, int32_t L) {
   Ctl* c = d.c;
+  LaneCtl* lc = &c->lane[lane];
   uint32_t n = 0;
   while (n < kMaxAdmit) {
-    const uint32_t slot = gcq::lfq::dequeue(d.subq[0], (gcq::lfq::NoProbe*)nullptr, true);
+    const uint32_t slot = gcq::lfq::dequeue(d.subq[lane], (gcq::lfq::NoProbe*)nullptr, true);
     if (slot == gcq::lfq::kBot) break;
     const Req* e = &d.desc[0][slot];
     const GraphDev& g = d.graphs[gcq::ld_rlx(&e->graph)];
@@ -212,24 +227,29 @@ __device__ void admit_bfs(const Dev& d, int32_t L) {
     gcq::st_rlx(&s->start, (uint32_t)(L + 2));  // cleared during L+1, source expanded at L+2
     gcq::st_rlx(&s->pad2[0], gcq::ld_rlx(&e->id));
     gcq::st_rlx(&d.bd.alive[slot * bk::kAliveStride], (uint32_t)(L + 2));
-    gcq::st_rlx(&d.slot_active[slot], 1u);
-    gcq::st_rlx(&c->adm_slot[n], slot);
+    gcq::st_rlx(&d.slot_active[slot], 1u + (uint32_t)lane);
+    gcq::st_rlx(&lc->adm_slot[n], slot);
     ++n;
-    gcq::faa_rlx(&c->active_bfs, 1);
+    gcq::faa_rlx(&lc->active, 1);
   }
-  gcq::st_rlx(&c->adm_n, n);
+  gcq::st_rlx(&lc->adm_n, n);
   // The host sets its stop word only once every request has completed, and
   // the gate wave reports it only after finding no doorbell entry left.
-  if (n == 0 && gcq::ld_rlx(&c->stop_seen) && gcq::ld_rlx(&c->active_bfs) == 0 &&
-      gcq::ld_rlx(&c->active_rt) == 0)
-    gcq::st_rlx(&c->exit, 1u);
+  const bool idle = n == 0 && gcq::ld_rlx(&c->stop_seen) && gcq::ld_rlx(&c->lane[0].active) == 0 &&
+                    gcq::ld_rlx(&c->lane[1].active) == 0 && gcq::ld_rlx(&c->active_rt) == 0;
+  // a lane whose barrier gave up (spin limit) takes the whole server down
+  const bool aborted = gcq::ld_rlx(&c->lane[0].bar.abort) | gcq::ld_rlx(&c->lane[1].bar.abort);
+  if (idle || aborted) {
+    gcq::st_rlx(&lc->quit, 1u);
+    gcq::st_rlx(&c->exit, 1u);  // for the gate and the RT waves
+  }
 }
 
 // Install one queued tile job, if any (lane 0 of an RT wave); true if one was.  The job's
 // fields land before the claim counter that carries its generation, and
 // that before the slot reads as active.
 __device__ bool admit_rt(const Dev& d) {
-  const uint32_t slot = gcq::lfq::dequeue(d.subq[1], (gcq::lfq::NoProbe*)nullptr, true);
+  const uint32_t slot = gcq::lfq::dequeue(d.subq[kLanes], (gcq::lfq::NoProbe*)nullptr, true);
   if (slot == gcq::lfq::kBot) return false;
   const Req* e = &d.desc[1][slot];
   RtJob* j = &d.jobs[slot];
@@ -254,25 +274,28 @@ __device__ bool admit_rt(const Dev& d) {
   return true;
 }
 
-// Block 0 after the barrier ending level L: queries with nothing discovered
-// at level L are finished.
-__device__ void done_scan(const Dev& d, int32_t L) {
+// The lane's first block after the barrier ending its level L: the lane's
+// queries with nothing discovered at level L are finished.
+__device__ void done_scan(const Dev& d, int lane, int32_t L) {
   for (uint32_t s = threadIdx.x; s < d.bd.nslots; s += blockDim.x) {
-    if (!gcq::ld_rlx(&d.slot_active[s])) continue;
+    if (gcq::ld_rlx(&d.slot_active[s]) != 1u + (uint32_t)lane) continue;
     const int32_t start = (int32_t)gcq::ld_rlx(&d.slots[s].start);
     if (start > L || (int32_t)gcq::ld_rlx(&d.bd.alive[s * bk::kAliveStride]) >= L + 1) continue;
     const uint64_t id = gcq::ld_rlx(&d.slots[s].pad2[0]);
     gcq::st_rlx(&d.slot_active[s], 0u);
-    gcq::faa_rlx(&d.c->active_bfs, -1);
+    gcq::faa_rlx(&d.c->lane[lane].active, -1);
     gcq::vm_wait();  // slot free and count down before the host can reuse the slot
     publish(d, id, GCQ_REQ_BFS, L + 1 - start, (int32_t)s, 0);
   }
 }
 
+// One BFS lane's sub-grid: blocks [first, first + lane_blocks[lane]).
 template <class F>
-__device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) {
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t nthreads = (uint64_t)d.bfs_blocks * blockDim.x;
+__device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_t first, uint32_t* stage) {
+  LaneCtl* const lc = &d.c->lane[lane];
+  const uint32_t nblocks = d.lane_blocks[lane];
+  const uint32_t tid = (blockIdx.x - first) * blockDim.x + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)nblocks * blockDim.x;
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
@@ -280,9 +303,9 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) 
     // (0 at the source) and visited bitmaps, and put their sources into this
     // level's output.  Their first expansion is a level later, after the
     // barrier's cache drop below.
-    const uint32_t na = gcq::ld_rlx(&d.c->adm_n);
+    const uint32_t na = gcq::ld_rlx(&lc->adm_n);
     for (uint32_t a = 0; a < na; ++a) {
-      const uint32_t s = gcq::ld_rlx(&d.c->adm_slot[a]);
+      const uint32_t s = gcq::ld_rlx(&lc->adm_slot[a]);
       const bk::Slot* sl = &d.slots[s];
       int32_t* dep = (int32_t*)gcq::ld_rlx(&sl->depth);
       uint32_t* vis = (uint32_t*)gcq::ld_rlx(&sl->visited);
@@ -297,16 +320,18 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, uint32_t* stage) 
     // the level after it, and every block drops its CU's and XCD's cached
     // lines after each barrier.
     (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage);
-    if (!grid_barrier(&d.c->bar, d.bfs_blocks, [&]() {
+    if (!grid_barrier(&lc->bar, nblocks, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
-          admit_bfs(d, L);
-        }))
+          admit_bfs(d, lane, L);
+        })) {
+      if (threadIdx.x == 0) gcq::st_rlx(&d.c->exit, 1u);  // the gate and the RT waves leave too
       break;
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (gcq::ld_rlx(&d.c->exit)) break;
-    if (blockIdx.x == 0) done_scan(d, L);
+    if (gcq::ld_rlx(&lc->quit)) break;
+    if (blockIdx.x == first) done_scan(d, lane, L);
   }
-  if (tid == 0) d.c->levels = L + 1;
+  if (tid == 0) lc->levels = L + 1;
 }
 
 __device__ void rt_part(const Dev& d) {
@@ -418,12 +443,15 @@ __device__ void rt_part(const Dev& d) {
   }
 }
 
-// Blocks [0, bfs_blocks) are the BFS sub-grid; the first wave after it is
-// the gate; the waves after that run tile jobs (when the server has a scene).
+// Blocks [0, bfs_blocks) are the BFS lanes' sub-grids; the first wave after
+// them is the gate; the waves after that run tile jobs (when the server has a
+// scene).
 template <class F>
-__global__ void k_serve(Frontiers<F> fq, Dev d) {
+__global__ void k_serve(Frontiers<F> fq0, Frontiers<F> fq1, Dev d) {
   __shared__ uint32_t stage[16][128];
-  if (blockIdx.x < d.bfs_blocks) bfs_part(fq, d, stage[(threadIdx.x / 64) & 15]);
+  uint32_t* const st = stage[(threadIdx.x / 64) & 15];
+  if (blockIdx.x < d.lane_blocks[0]) bfs_part(fq0, d, 0, 0u, st);
+  else if (blockIdx.x < d.bfs_blocks) bfs_part(fq1, d, 1, d.lane_blocks[0], st);
   else if (blockIdx.x == d.bfs_blocks && threadIdx.x < 64) gate(d);
   else if (d.rt_workers) rt_part(d);
 }
@@ -477,7 +505,7 @@ struct Pending {
 };
 struct InFlight {
   gcq_request r;
-  int32_t index, slot;
+  int32_t index, slot, lane;
   uint64_t off1, len1, off2, len2;
   uint32_t nv;
   double admit_ms;
@@ -495,7 +523,9 @@ struct gcq_server {
   int32_t W = 0, H = 0;
   gcq_server_config cfg{};
   hipStream_t stream = nullptr;
-  gcqf::FrontierSet fs;
+  gcqf::FrontierSet fs[sv::kLanes];  // per BFS lane
+  uint64_t lane_cap[sv::kLanes] = {0, 0};  // frontier ring capacity per lane
+  std::vector<int32_t> glane;        // the BFS lane of each graph
   uint32_t vbits = 1;
   // device
   bk::Slot* slots = nullptr;
@@ -506,8 +536,8 @@ struct gcq_server {
   char* rings = nullptr;
   uint64_t ring_stride = 0;
   sv::Ctl* ctl = nullptr;
-  char* subq = nullptr;        // the two submission queues (G-LFQ rings): BFS, RT
-  uint64_t subq_off = 0;       // offset of the RT queue
+  char* subq = nullptr;        // the submission queues (G-LFQ rings): BFS lane 0, BFS lane 1, RT
+  uint64_t subq_off[3] = {0, 0, 0};
   sv::Req* desc = nullptr;     // request descriptors: bfs_slots BFS, then rt_slots RT
   char* arena_base = nullptr;  // in-flight request state, budget_bytes
   sv::Arena arena;
@@ -521,7 +551,7 @@ struct gcq_server {
   bool running = false, stopped = false;
   uint64_t sub_tail = 0, comp_head = 0;
   std::vector<int32_t> free_bfs, free_rt;
-  uint64_t frontier_used = 0;
+  uint64_t frontier_used[sv::kLanes] = {0, 0};
   std::deque<sv::Pending> pending;
   std::map<uint64_t, sv::InFlight> inflight;  // by internal id (submission index)
   std::map<uint64_t, sv::Kept> kept;          // by caller id
@@ -544,7 +574,7 @@ void release_all(gcq_server* s) {
     if (p) (void)hipFree(p);
   for (void* p : {(void*)s->sub, (void*)s->comp, (void*)s->stop})
     if (p) (void)hipHostFree(p);
-  s->fs.release();
+  for (auto& f : s->fs) f.release();
 }
 
 uint64_t rt_bytes(int32_t w, int32_t h) { return (uint64_t)w * h * (12ull + sizeof(rtk::RayRec)); }
@@ -558,7 +588,7 @@ int check_request(gcq_server* s, const gcq_request* r) {
     const uint64_t nv = (uint64_t)s->graphs[r->graph]->host.nv;
     GCQ_REQUIRE(4 * nv + vis_bytes(nv) <= s->cfg.budget_bytes, "query needs %llu bytes, above the budget",
                 (unsigned long long)(4 * nv + vis_bytes(nv)));
-    GCQ_REQUIRE(nv <= (1ull << s->cfg.frontier_log2), "graph larger than the frontier ring");
+    GCQ_REQUIRE(nv <= s->lane_cap[s->glane[r->graph]], "graph larger than its lane's frontier ring");
   } else if (r->type == GCQ_REQ_RT) {
     GCQ_REQUIRE(s->scene, "server has no scene");
     GCQ_REQUIRE(r->w > 0 && r->h > 0 && (uint64_t)r->w * r->h <= sv::kMaxTilePixels, "tile must hold 1..65536 px");
@@ -585,7 +615,8 @@ int try_admit(gcq_server* s) {
     if (r.type == GCQ_REQ_BFS) {
       gcq_graph* g = s->graphs[r.graph];
       const uint32_t nv = (uint32_t)g->host.nv;
-      if (s->free_bfs.empty() || s->frontier_used + nv > (1ull << s->cfg.frontier_log2)) break;
+      const int32_t lane = s->glane[r.graph];
+      if (s->free_bfs.empty() || s->frontier_used[lane] + nv > s->lane_cap[lane]) break;
       sv::Arena& a1 = r.keep ? s->keep : s->arena;
       if (!a1.alloc(4ull * nv, &f.off1)) {
         GCQ_REQUIRE(!r.keep, "keep_bytes exhausted by kept results");
@@ -599,8 +630,9 @@ int try_admit(gcq_server* s) {
       f.len2 = vis_bytes(nv);
       f.nv = nv;
       f.slot = s->free_bfs.back();
+      f.lane = lane;
       s->free_bfs.pop_back();
-      s->frontier_used += nv;
+      s->frontier_used[lane] += nv;
       e.graph = r.graph;
       e.src = (r.src >= 0 && r.src < (int32_t)nv) ? r.src : 0;  // bfs_bench.cpp:555
       e.buf = (uint64_t)((r.keep ? s->keep_base : s->arena_base) + f.off1);
@@ -662,7 +694,7 @@ void collect(gcq_server* s) {
     c.done_ms = s->now_ms();
     if (f.r.type == GCQ_REQ_BFS) {
       s->free_bfs.push_back(f.slot);
-      s->frontier_used -= f.nv;
+      s->frontier_used[f.lane] -= f.nv;
     } else {
       s->free_rt.push_back(f.slot);
     }
@@ -673,6 +705,19 @@ void collect(gcq_server* s) {
     s->ready_index.push_back(f.index);
     s->inflight.erase(it);
   }
+}
+
+// While requests are in flight the persistent kernel ends only after a lane's
+// barrier gave up; report that instead of waiting for completions that will
+// not come.
+int check_kernel_alive(gcq_server* s) {
+  const hipError_t q = hipStreamQuery(s->stream);
+  if (q == hipErrorNotReady) return GCQ_OK;
+  if (q != hipSuccess) GCQ_HIP(q);
+  s->running = false;  // nothing runs on the server's memory any more
+  s->stopped = true;
+  gcqh::set_error("server kernel ended with requests in flight (a BFS barrier wait exceeded its spin limit)");
+  return GCQ_ETIMEOUT;
 }
 
 int submit_at(gcq_server* s, const gcq_request* r, int32_t index) {
@@ -714,6 +759,10 @@ extern "C" int gcq_server_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_
   GCQ_REQUIRE(cfg->pop_width >= 1 && cfg->pop_width <= 64 && cfg->flush_at >= 1 && cfg->flush_at <= 64,
               "bad pop_width / flush_at");
   GCQ_REQUIRE(cfg->max_requests >= 1 && cfg->budget_bytes >= 4096 && cfg->max_depth >= 0, "bad limits");
+  GCQ_REQUIRE(cfg->small_log2 <= 30, "small_log2 must be at most 30");
+  GCQ_REQUIRE(!cfg->small_log2 || (cfg->small_threads >= cfg->block && cfg->small_threads % cfg->block == 0 &&
+                                   cfg->small_threads < cfg->bfs_threads),
+              "the small-graph lane needs small_threads: a multiple of block below bfs_threads");
   GCQ_REQUIRE(!scene || (frame_w > 0 && frame_h > 0), "bad frame");
   gcq_server* s = new gcq_server();
   s->graphs.assign(graphs, graphs + ngraphs);
@@ -730,6 +779,15 @@ extern "C" int gcq_server_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_
     max_nv = std::max(max_nv, g->host.nv);
   }
   while ((1ll << s->vbits) < (int64_t)max_nv) ++s->vbits;
+  // Lanes: graphs of at most 2^small_log2 vertices go to lane 1.  Each lane's
+  // frontier ring holds the V sum of its in-flight queries.
+  const bool two = cfg->small_log2 != 0;
+  for (gcq_graph* g : s->graphs)
+    s->glane.push_back(two && (uint64_t)g->host.nv <= (1ull << cfg->small_log2) ? 1 : 0);
+  s->lane_cap[0] = 1ull << cfg->frontier_log2;
+  s->lane_cap[1] = two ? std::min<uint64_t>(s->lane_cap[0], gcqf::pow2_at_least((uint64_t)cfg->bfs_slots
+                                                                                 << cfg->small_log2))
+                       : 0;
   if ((uint64_t)cfg->bfs_slots > (1ull << (31 - s->vbits))) {
     delete s;
     GCQ_REQUIRE(false, "%u BFS slots do not fit beside %u-bit vertices", cfg->bfs_slots, s->vbits);
@@ -755,7 +813,11 @@ extern "C" int gcq_server_start(gcq_server* s) {
     if (rc) return rc;
   }
   GCQ_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  int rc = s->fs.ensure(c.kind, 1ull << c.frontier_log2, c.bfs_threads);
+  const uint32_t lane_threads[2] = {c.bfs_threads - (c.small_log2 ? c.small_threads : 0u),
+                                    c.small_log2 ? c.small_threads : 0u};
+  int rc = GCQ_OK;
+  for (int l = 0; l < sv::kLanes && rc == GCQ_OK; ++l)
+    if (lane_threads[l]) rc = s->fs[l].ensure(c.kind, s->lane_cap[l], lane_threads[l]);
   if (rc) return rc;
   const uint32_t S = c.bfs_slots, J = c.rt_slots;
   GCQ_HIP(hipMalloc(&s->slots, sizeof(bk::Slot) * S));
@@ -766,11 +828,16 @@ extern "C" int gcq_server_start(gcq_server* s) {
   s->ring_stride = (gcq::lfq::ring_bytes(sv::kMaxTilePixels) + 255) & ~255ull;
   GCQ_HIP(hipMalloc(&s->rings, s->ring_stride * J));
   GCQ_HIP(hipMalloc(&s->ctl, sizeof(sv::Ctl)));
-  // Submission queues: a slot is queued at most once, so capacity = slots.
-  const uint32_t qn[2] = {gcqf::pow2_at_least(std::max<uint64_t>(S, 64)),
+  // Submission queues (BFS lane 0, BFS lane 1, RT): a slot is queued at
+  // most once, so capacity = slots.
+  const uint32_t qn[3] = {gcqf::pow2_at_least(std::max<uint64_t>(S, 64)), gcqf::pow2_at_least(std::max<uint64_t>(S, 64)),
                           gcqf::pow2_at_least(std::max<uint64_t>(J, 64))};
-  s->subq_off = (gcq::lfq::ring_bytes(qn[0]) + 255) & ~255ull;
-  GCQ_HIP(hipMalloc(&s->subq, s->subq_off + gcq::lfq::ring_bytes(qn[1])));
+  uint64_t qbytes = 0;
+  for (int q = 0; q < 3; ++q) {
+    s->subq_off[q] = qbytes;
+    qbytes += (gcq::lfq::ring_bytes(qn[q]) + 255) & ~255ull;
+  }
+  GCQ_HIP(hipMalloc(&s->subq, qbytes));
   GCQ_HIP(hipMalloc(&s->desc, sizeof(sv::Req) * (S + J)));
   GCQ_HIP(hipMalloc(&s->arena_base, c.budget_bytes));
   s->arena.init(c.budget_bytes);
@@ -787,7 +854,7 @@ extern "C" int gcq_server_start(gcq_server* s) {
   std::vector<sv::GraphDev> gd(std::max<size_t>(1, s->graphs.size()));
   for (size_t i = 0; i < s->graphs.size(); ++i)
     gd[i] = sv::GraphDev{(uint64_t)s->graphs[i]->d_off, (uint64_t)s->graphs[i]->d_idx,
-                         (uint32_t)s->graphs[i]->host.nv, 0, 0};
+                         (uint32_t)s->graphs[i]->host.nv, (uint32_t)s->glane[i], 0};
   GCQ_HIP(hipMemcpy(s->dgraphs, gd.data(), sizeof(sv::GraphDev) * gd.size(), hipMemcpyHostToDevice));
   auto zero = [&](void* p, uint64_t bytes) {
     hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(bytes / 8)), dim3(256), 0, s->stream, (uint64_t*)p,
@@ -798,12 +865,13 @@ extern "C" int gcq_server_start(gcq_server* s) {
   zero(s->slot_active, (4ull * S + 7) & ~7ull);
   zero(s->jobs, sizeof(sv::RtJob) * J);
   zero(s->ctl, sizeof(sv::Ctl));
-  s->fs.reset(s->stream);
+  for (auto& f : s->fs)
+    if (f.kind >= 0) f.reset(s->stream);
   hipLaunchKernelGGL(sv::k_init_rings, dim3(gcqf::grid_of(2ull * sv::kMaxTilePixels)), dim3(256), 0, s->stream,
                      s->rings, s->ring_stride, sv::kMaxTilePixels, J);
-  for (int q = 0; q < 2; ++q)
+  for (int q = 0; q < 3; ++q)
     hipLaunchKernelGGL(sv::k_init_rings, dim3(gcqf::grid_of(2ull * qn[q])), dim3(256), 0, s->stream,
-                       s->subq + (q ? s->subq_off : 0), (uint64_t)0, qn[q], 1u);
+                       s->subq + s->subq_off[q], (uint64_t)0, qn[q], 1u);
   zero(s->desc, sizeof(sv::Req) * (S + J));
   GCQ_HIP(hipGetLastError());
   GCQ_HIP(hipStreamSynchronize(s->stream));
@@ -817,8 +885,7 @@ extern "C" int gcq_server_start(gcq_server* s) {
   d.rings = s->rings;
   d.ring_stride = s->ring_stride;
   d.sub = s->sub;
-  d.subq[0] = (gcq::lfq::Ring*)s->subq;
-  d.subq[1] = (gcq::lfq::Ring*)(s->subq + s->subq_off);
+  for (int q = 0; q < 3; ++q) d.subq[q] = (gcq::lfq::Ring*)(s->subq + s->subq_off[q]);
   d.desc[0] = s->desc;
   d.desc[1] = s->desc + S;
   d.comp = s->comp;
@@ -830,6 +897,8 @@ extern "C" int gcq_server_start(gcq_server* s) {
   d.max_depth = c.max_depth;
   d.njobs = J;
   d.bfs_blocks = c.bfs_threads / c.block;
+  d.lane_blocks[0] = lane_threads[0] / c.block;
+  d.lane_blocks[1] = lane_threads[1] / c.block;
   d.max_requests = c.max_requests;
   d.rt_workers = s->scene ? 1u : 0u;
   // The gate wave leads the first block after the BFS sub-grid; a server
@@ -838,14 +907,16 @@ extern "C" int gcq_server_start(gcq_server* s) {
   int dev = 0, cus = 0, per_cu = 0;
   GCQ_HIP(hipGetDevice(&dev));
   GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  rc = s->fs.dispatch([&](const auto& fq) {
+  rc = s->fs[0].dispatch([&](const auto& fq) {
     using F = std::decay_t<decltype(fq.q[0])>;
+    // lane 1 uses the same queue kind; without it, its (unused) view is lane 0's
+    const gcqf::Frontiers<F> fq1 = lane_threads[1] ? s->fs[1].template view<F>() : fq;
     GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sv::k_serve<F>, c.block, 0));
     if ((int64_t)grid > (int64_t)per_cu * cus) {
       gcqh::set_error("server grid of %u blocks does not fit resident (%d)", grid, per_cu * cus);
       return GCQ_EINVAL;
     }
-    hipLaunchKernelGGL(sv::k_serve<F>, dim3(grid), dim3(c.block), 0, s->stream, fq, d);
+    hipLaunchKernelGGL(sv::k_serve<F>, dim3(grid), dim3(c.block), 0, s->stream, fq, fq1, d);
     GCQ_HIP(hipGetLastError());
     return GCQ_OK;
   });
@@ -887,6 +958,7 @@ extern "C" int gcq_server_stop(gcq_server* s) {
     collect(s);
     int rc = try_admit(s);
     if (rc) return rc;
+    if (!s->inflight.empty() && (rc = check_kernel_alive(s)) != GCQ_OK) return rc;
     if (s->now_ms() > deadline) {
       gcqh::set_error("server: requests still in flight after %.0f ms", kStopTimeoutMs);
       return GCQ_ETIMEOUT;
@@ -908,7 +980,7 @@ extern "C" int gcq_server_stop(gcq_server* s) {
   s->stopped = true;
   sv::Ctl h;
   GCQ_HIP(hipMemcpy(&h, s->ctl, sizeof h, hipMemcpyDeviceToHost));
-  if (h.bar.abort) {
+  if (h.lane[0].bar.abort || h.lane[1].bar.abort) {
     gcqh::set_error("server aborted: a BFS barrier wait exceeded its spin limit");
     return GCQ_ETIMEOUT;
   }
@@ -935,8 +1007,10 @@ extern "C" int gcq_server_run_trace(gcq_server* s, const gcq_request* reqs, int3
   int rc = gcq_server_start(s);
   if (rc) return rc;
   int32_t next = 0, done = 0;
+  uint32_t spins = 0;
   const double deadline = reqs[n - 1].arrival_ms + kStopTimeoutMs;
   while (done < n) {
+    if ((++spins & 255u) == 0 && !s->inflight.empty() && (rc = check_kernel_alive(s)) != GCQ_OK) return rc;
     const double now = s->now_ms();
     while (next < n && reqs[next].arrival_ms <= now) {
       rc = submit_at(s, &reqs[next], next);

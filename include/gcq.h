@@ -443,6 +443,9 @@ typedef struct gcq_server_config {
   uint64_t budget_bytes;   /* device memory for request state: depth arrays, ray records, images */
   uint64_t keep_bytes;     /* separate device memory for results kept for gcq_server_fetch */
   uint32_t max_requests;   /* submission / completion ring entries for the server's lifetime */
+  uint32_t small_log2;     /* small-graph lane: queries on graphs of at most 2^small_log2 vertices run in a
+                              second BFS sub-grid with its own level barrier (0 = one lane for all) */
+  uint32_t small_threads;  /* the second lane's threads, taken out of bfs_threads (multiple of block) */
   uint32_t pad;
 } gcq_server_config;
 void gcq_server_config_default(gcq_server_config* cfg);

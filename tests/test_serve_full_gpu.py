@@ -1,7 +1,9 @@
 """The served mix at its benchmark scale: resident R-MAT graphs of scales
 14..22 (edge factor 16, seed 11), the 1920x1080 64-sphere scene, a 64 GB
 request budget and the benchmark's server configuration (a 2^28-slot
-frontier ring, 229,376 BFS + 32,768 RT threads in blocks of 512).  About 60
+frontier ring, 229,376 BFS + 32,768 RT threads in blocks of 512), with one
+BFS lane and with 32,768 of the BFS threads in a second lane for the graphs
+of at most 2^18 vertices.  About 60
 requests, 90% BFS queries on sources in each graph's giant component, 10%
 256 x 256 tiles, served both as a Poisson trace at 150 requests/s and as one
 burst.  Every BFS depth array equals the serial BFS oracle's bit for bit;
@@ -16,6 +18,7 @@ FRAME = (1920, 1080)
 SCALES = list(range(14, 23))
 SERVER = dict(bfs_threads=229376, block=512, rt_threads=32768, frontier_log2=28, budget_bytes=64 << 30,
               keep_bytes=4 << 30)
+LANES = {1: {}, 2: dict(small_log2=18, small_threads=32768)}
 
 
 @pytest.fixture(scope="module")
@@ -33,8 +36,9 @@ def big_world(gcq, orc):
         g.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2], ids=["one_lane", "two_lanes"])
 @pytest.mark.parametrize("rate", [150.0, 1e9], ids=["poisson150", "burst"])
-def test_cfg5_scale_mix_is_exact(gcq, orc, big_world, rate):
+def test_cfg5_scale_mix_is_exact(gcq, orc, big_world, rate, lanes):
     from gcq_amd import serve
     graphs, scene, csrs, full = big_world
     nvs = [len(c[0]) - 1 for c in csrs]
@@ -45,9 +49,9 @@ def test_cfg5_scale_mix_is_exact(gcq, orc, big_world, rate):
           f"scales {sorted({SCALES[r['graph']] for r in rows if r['type'] == 0})}", flush=True)
     assert any(r["type"] == 0 and SCALES[r["graph"]] == 22 for r in rows)
     assert any(r["type"] == 1 for r in rows)
-    with serve.Server(graphs, scene, FRAME, **SERVER) as srv:
+    with serve.Server(graphs, scene, FRAME, **SERVER, **LANES[lanes]) as srv:
         comps = srv.run_trace(rows)
-        print("[serve full]", serve.latency_summary(comps), flush=True)
+        print(f"[serve full] lanes={lanes}", serve.latency_summary(comps), flush=True)
         assert len(comps) == len(rows)
         for r, c in zip(rows, comps):
             assert c["status"] == 0 and c["id"] == r["id"] and c["type"] == r["type"]

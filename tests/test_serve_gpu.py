@@ -46,22 +46,29 @@ def _check_results(orc, srv, rows, comps, graphs, scene, csrs, full_frame=None):
                 assert float(np.abs(img - ref).max()) <= 1e-5
 
 
-def test_mix_equals_alone(gcq, orc, world):
+# Two lanes: queries on graphs of at most 2^11 vertices (R-MAT scales 9 and
+# 11) run in a second BFS sub-grid of 4,096 threads with its own barrier.
+LANES = {1: {}, 2: dict(small_log2=11, small_threads=4096)}
+
+
+@pytest.mark.parametrize("lanes", [1, 2], ids=["one_lane", "two_lanes"])
+def test_mix_equals_alone(gcq, orc, world, lanes):
     from gcq_amd import serve
     graphs, scene, csrs = world
     nvs = [len(c[0]) - 1 for c in csrs]
     rows = serve.poisson_trace(150, 3000.0, nvs, FRAME, rt_fraction=0.1, seed=3, keep=1)
     full, _ = orc.rt_render(*scene.arrays(), FRAME[0], FRAME[1])
-    print(f"[serve mix] {len(rows)} requests", flush=True)
+    print(f"[serve mix] {len(rows)} requests, {lanes} lane(s)", flush=True)
     with serve.Server(graphs, scene, FRAME, budget_bytes=1 << 30, keep_bytes=1 << 30, bfs_threads=16384,
-                      rt_threads=8192, frontier_log2=22) as srv:
+                      rt_threads=8192, frontier_log2=22, **LANES[lanes]) as srv:
         comps = srv.run_trace(rows)
         print("[serve mix]", serve.latency_summary(comps), flush=True)
         _check_results(orc, srv, rows, comps, graphs, scene, csrs, full)
 
 
+@pytest.mark.parametrize("lanes", [1, 2], ids=["one_lane", "two_lanes"])
 @pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["gwfq", "glfq", "wfq", "sfq"])
-def test_burst_with_tight_limits(gcq, orc, world, kind):
+def test_burst_with_tight_limits(gcq, orc, world, kind, lanes):
     # Everything arrives at once; 3 BFS slots, 2 tile slots and a budget of
     # about two of the largest queries force requests to wait for room
     # (kept results live in their own keep_bytes arena).
@@ -70,9 +77,10 @@ def test_burst_with_tight_limits(gcq, orc, world, kind):
     nvs = [len(c[0]) - 1 for c in csrs]
     rows = serve.poisson_trace(40, 1e9, nvs, FRAME, tile=128, rt_fraction=0.25, seed=kind, keep=1)
     budget = 2 * 4 * max(nvs) + 2 * 128 * 128 * 76 + 4096
-    print(f"[serve burst] kind={kind}", flush=True)
+    print(f"[serve burst] kind={kind} lanes={lanes}", flush=True)
     with serve.Server(graphs, scene, FRAME, kind=kind, budget_bytes=budget, keep_bytes=1 << 28, bfs_slots=3,
-                      rt_slots=2, bfs_threads=4096, rt_threads=4096, frontier_log2=20) as srv:
+                      rt_slots=2, bfs_threads=8192 if lanes == 2 else 4096, rt_threads=4096, frontier_log2=20,
+                      **LANES[lanes]) as srv:
         comps = srv.run_trace(rows)
         _check_results(orc, srv, rows, comps, graphs, scene, csrs)
     # at no time more than 3 BFS queries / 2 tiles between admission and completion
