@@ -286,8 +286,8 @@ def profile_run(a):
                                    graph_sources=lambda gi, i: graphs[gi].pick_source(11, i))
         with rt.Scene.preset(rt.RANDOM, 64, 3) as scene:
             for _ in range(a.runs):
-                with serve.Server(graphs, scene, (1920, 1080), bfs_threads=229376, block=512, rt_threads=32768,
-                                  frontier_log2=28, small_log2=18, small_threads=32768) as srv:
+                with serve.Server(graphs, scene, (1920, 1080), bfs_threads=245760, block=512, rt_threads=16384,
+                                  frontier_log2=28, small_log2=18, small_threads=8192) as srv:
                     res = serve.latency_summary(srv.run_trace(rows))
         out.update(res)
         for gr in graphs:
@@ -435,9 +435,11 @@ def secondary(g, with_cpu):
     rows = serve.poisson_trace(300, lam, nvs, frame, rt_fraction=0.1, seed=5, graph_sources=pick)
     burst = serve.poisson_trace(300, 1e9, nvs, frame, rt_fraction=0.1, seed=6, graph_sources=pick)
     # two BFS lanes: graphs of at most 2^18 vertices (scales 14..18) get
-    # 32,768 of the BFS threads and their own level barrier
-    scfg = dict(bfs_threads=229376, block=512, rt_threads=32768, frontier_log2=28, small_log2=18,
-                small_threads=32768)
+    # 8,192 of the BFS threads and their own level barrier; tiles 16,384
+    # threads (a sweep of splits on MI355X: this one keeps the one-lane p99
+    # and cuts p50 about 10x)
+    scfg = dict(bfs_threads=245760, block=512, rt_threads=16384, frontier_log2=28, small_log2=18,
+                small_threads=8192)
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         comps = srv.run_trace(rows)
         s_trace = serve.latency_summary(comps)
