@@ -98,7 +98,17 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
   uint32_t sc = 0;  // staged vertices (wave-uniform)
   uint64_t end = 0;
   bool one_round = false;
+  // One-round levels hand out their tickets by grid position, spread over
+  // the waves: lane l of wave w takes h0 + l * W + w (W waves), so a level
+  // of n items gives each wave about n / W of them, one 64-edge round, and
+  // every ticket in [h0, h0 + 64 W) goes to exactly one lane.  Its slot
+  // word is read beside the level's end.
+  const uint64_t tix = h0 + (uint64_t)lane * (grid_lanes / 64) + tid / 64;
+  uint64_t first_word = 0;
   if constexpr (kCounted) {
+    if constexpr (F::kHeadFromDrain) {
+      if (h0 != kNoHead) first_word = qin.peek(tix);
+    }
     end = qin.level_end();
     if (end_out) *end_out = end;
     one_round = h0 != kNoHead && end >= h0 && end - h0 <= grid_lanes;
@@ -107,7 +117,13 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
     uint32_t u = 0;
     bool got;
     if constexpr (kCounted) {
-      const int st = qin.pop_counted(tid, end, &u);
+      int st;
+      if constexpr (F::kHeadFromDrain) {
+        // a one-round level takes its tickets by grid position, with no FAA on Head
+        st = one_round ? qin.take(tid, tix, end, first_word, &u) : qin.pop_counted(tid, end, &u);
+      } else {
+        st = qin.pop_counted(tid, end, &u);
+      }
       if (__ballot(st >= 0) == 0) break;  // every lane drew a ticket past the level's end
       got = st > 0;
     } else {

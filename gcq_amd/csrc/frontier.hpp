@@ -30,6 +30,12 @@ constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
 // kHeadFromDrain: drained() leaves Head equal to the level's end, so a ring's
 // Head at the start of the next level it feeds is that end, known to every
 // wave (two frontiers alternate, so it is the end of two levels before).
+// Such a ring also offers peek(T) / take(T, end, word): when a level's
+// tickets [h0, end) fit in the grid's lanes, each lane takes one ticket of
+// [h0, h0 + lanes) fixed by its grid position, without an FAA on Head (each
+// ticket is taken once, as the FAAs would have handed them out), and its
+// slot read goes out together with the read of the level's end.  Head stays
+// at h0 and is set to the end by drained(), as after FAA pops.
 struct FGwf {
   static constexpr int kQueues = 2;
   static constexpr bool kCounted = true;
@@ -47,8 +53,20 @@ struct FGwf {
     RingOps<> o(r, tid, &np);
     const uint64_t T = o.cnt_of(gcq::wave_faa(head(r), (uint64_t)1 << r->g.thr_bits));
     if (T >= end) return -1;
+    return take(tid, T, end, peek(T), v);
+  }
+  __device__ __forceinline__ uint64_t peek(uint64_t T) const {
+    gcq::gwf::NoProbe np;
+    gcq::gwf::RingOps<> o(r, 0, &np);
+    return gcq::ld_rlx(&gcq::gwf::slots(r)[o.slot(T)]);
+  }
+  // Ticket T with its slot word E as first read: as pop_counted after its FAA.
+  __device__ __forceinline__ int take(uint32_t tid, uint64_t T, uint64_t end, uint64_t E, uint32_t* v) const {
+    using namespace gcq::gwf;
+    if (T >= end) return -1;
+    NoProbe np;
+    RingOps<> o(r, tid, &np);
     const uint32_t j = o.slot(T), tcyc = o.cyc(T);
-    uint64_t E = gcq::ld_rlx(&slots(r)[j]);
     for (uint32_t w = 0; (e_cyc(E) != tcyc || e_idx(E) >= kBot) && w < gcq::kSlotWaitSpins; ++w) {
       __builtin_amdgcn_s_sleep(1);
       E = gcq::ld_rlx(&slots(r)[j]);
@@ -91,14 +109,23 @@ struct FLfq {
   __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::lfq::tail(r)); }
   // 1 = *v popped, 0 = this ticket holds no item (skip it), -1 = the level
   // is drained.
-  __device__ __forceinline__ int pop_counted(uint32_t, uint64_t end, uint32_t* v) const {
+  __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
+    const uint64_t T = gcq::wave_faa(gcq::lfq::head(r), (uint64_t)1);
+    if (T >= end) return -1;
+    return take(tid, T, end, peek(T), v);
+  }
+  __device__ __forceinline__ uint64_t peek(uint64_t T) const {
     using namespace gcq::lfq;
-    const uint64_t T = gcq::wave_faa(head(r), (uint64_t)1);
+    const Geometry g = r->g;
+    return gcq::ld_rlx(slots(r) + slot_of(T, g.slots_log2, g.remap_log2));
+  }
+  // Ticket T with its slot word E as first read: as pop_counted after its FAA.
+  __device__ __forceinline__ int take(uint32_t, uint64_t T, uint64_t end, uint64_t E, uint32_t* v) const {
+    using namespace gcq::lfq;
     if (T >= end) return -1;
     const Geometry g = r->g;
     const uint32_t tcyc = cycle_of(T, g.slots_log2);
     uint64_t* const e = slots(r) + slot_of(T, g.slots_log2, g.remap_log2);
-    uint64_t E = gcq::ld_rlx(e);
     // Written before the barrier; the wait only guards against a ticket an
     // enqueuer abandoned.
     for (uint32_t w = 0; (e_cyc(E) != tcyc || e_idx(E) >= kBot) && w < gcq::kSlotWaitSpins; ++w) {
@@ -133,13 +160,21 @@ struct FSfq {
   // item and the "full" phase id; the pop reads the item and hands the slot
   // to the next lap, as a blocking dequeue does after its wait.
   __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::sfq::tail(q)); }
-  __device__ __forceinline__ int pop_counted(uint32_t, uint64_t end, uint32_t* v) const {
+  __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
+    const uint64_t T = gcq::wave_faa(gcq::sfq::head(q), (uint64_t)1);
+    if (T >= end) return -1;
+    return take(tid, T, end, peek(T), v);
+  }
+  __device__ __forceinline__ uint64_t peek(uint64_t T) const {
+    return gcq::ld_rlx(&gcq::sfq::slots(q)[T & (q->g.qlen - 1)].id);
+  }
+  // Ticket T with its slot's phase id as first read: as pop_counted after its FAA.
+  __device__ __forceinline__ int take(uint32_t, uint64_t T, uint64_t end, uint64_t first_id, uint32_t* v) const {
     using namespace gcq::sfq;
-    const uint64_t T = gcq::wave_faa(head(q), (uint64_t)1);
     if (T >= end) return -1;
     Slot* s = &slots(q)[T & (q->g.qlen - 1)];
     const uint64_t id = 2ull * (T >> q->g.qlen_log2) + 1ull;
-    bool ready = gcq::ld_rlx(&s->id) == id;
+    bool ready = first_id == id;
     for (uint32_t w = 0; !ready && w < gcq::kSlotWaitSpins; ++w) {
       __builtin_amdgcn_s_sleep(1);
       ready = gcq::ld_rlx(&s->id) == id;
