@@ -44,10 +44,20 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
   uint32_t* st = stage[(threadIdx.x / 64) & 15];
+  uint64_t prev_end[2] = {kNoHead, kNoHead};  // per frontier: its end when last drained
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
-    const uint32_t dw = wave_sum32(expand<F, false>(fq.in(L), fq.out(L), d, L, tid, st));
+    // The frontier drained at the last barrier is this level's output: its
+    // Head goes back to that level's end (the FAA pops overshoot it), off
+    // the barrier's path; the next barrier orders the store before the level
+    // that reads it.
+    if constexpr (F::kQueues == 2)
+      if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
+    uint64_t end = 0;
+    const uint32_t dw = wave_sum32(expand<F, false>(fq.in(L), fq.out(L), d, L, tid, st,
+                                                    F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end));
+    prev_end[L & 1] = end;
     if (lane_id() == 0 && dw) gcq::faa_rlx(&ctl->next[L % 3], dw);
     if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();

@@ -78,7 +78,6 @@ constexpr int32_t kProbeEdges = 256;
 // find an item, and a wave leaves after its first batch instead of paying
 // one more FAA round trip to learn that.  h0 = kNoHead when the level's
 // starting Head is not known.
-constexpr uint64_t kNoHead = ~0ull;
 
 // kBitmap (GCQ_BFS_BITMAP): a vertex is claimed in the visited bitmap `vis`
 // (one bit per vertex, 1,024 vertices per 128-B line) instead of by a CAS on

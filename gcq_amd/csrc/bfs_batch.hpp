@@ -45,17 +45,45 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int lane) {
 // slots), so they are read at device scope; otherwise (one batch per launch)
 // slots are fixed and a depth only goes from -1 to a level, so the visited
 // filter and the slot fields are read through the caches.
+//
+// The kernels are level-synchronous (a grid barrier ends every level, and
+// every push of a level lands in its output queue), so the input queue is
+// drained by counted pops (F::pop_counted: one FAA on Head against the
+// level's end, read at its start, and no claim reads).  The caller sets a
+// drained queue's Head back to its end (F::drained) and, when the queue kind
+// keeps Head there (kHeadFromDrain), passes that end as h0 when the queue
+// next feeds a level: a level whose tickets [h0, end) fit in the grid's
+// popping lanes (grid_waves x pop_width) then takes them by grid position,
+// lane l of wave w ticket h0 + l * grid_waves + w, with no FAA on Head.
+// end_out receives the level's end.
 template <class F, bool kLive = true>
-__device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L, uint32_t tid, uint32_t* stage) {
+__device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L, uint32_t tid, uint32_t* stage,
+                           uint64_t h0, uint64_t grid_waves, uint64_t* end_out) {
   auto rd = [](const auto* p) { return kLive ? gcq::ld_rlx(p) : gcq::ld_cached(p); };
   const uint32_t lane = lane_id();
   const uint32_t vmask = (1u << d.vbits) - 1u;
   uint32_t sc = 0, disc_n = 0;
+  const bool pops = lane < d.pop_width;
+  const uint64_t tix = h0 + (uint64_t)lane * grid_waves + tid / 64;
+  uint64_t first_word = 0;
+  if constexpr (F::kHeadFromDrain) {
+    if (h0 != kNoHead && pops) first_word = qin.peek(tix);
+  }
+  const uint64_t end = qin.level_end();
+  *end_out = end;
+  const bool one_round = F::kHeadFromDrain && h0 != kNoHead && end >= h0 && end - h0 <= grid_waves * d.pop_width;
   for (;;) {
     uint32_t item = 0;
-    bool got = false;
-    if (lane < d.pop_width) got = qin.pop(tid, &item);
-    if (__ballot(got) == 0) break;  // every popping lane saw the level's queue empty
+    int st = -1;
+    if (pops) {
+      if constexpr (F::kHeadFromDrain) {
+        st = one_round ? qin.take(tid, tix, end, first_word, &item) : qin.pop_counted(tid, end, &item);
+      } else {
+        st = qin.pop_counted(tid, end, &item);
+      }
+    }
+    if (__ballot(st >= 0) == 0) break;  // every popping lane drew a ticket past the level's end
+    const bool got = st > 0;
     uint32_t slot = 0;
     int32_t beg = 0, deg = 0, dn = 0;
     uint64_t idxp = 0, depp = 0, vis = 0;
@@ -145,6 +173,7 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
       disc_n += disc ? 1u : 0u;
       while (sc >= d.flush_at) flush(qout, tid, stage, sc, sc < 64u ? sc : 64u, d.err);
     }
+    if (one_round) break;  // every ticket of the level was handed out in this round
   }
   if (sc) flush(qout, tid, stage, sc, sc, d.err);
   return disc_n;

@@ -19,6 +19,8 @@ namespace gcqf {
 using gcq::lane_id;
 
 constexpr uint64_t kBarrierSpinLimit = 1ull << 25;
+// A level's starting Head when it is not known (see kHeadFromDrain below).
+constexpr uint64_t kNoHead = ~0ull;
 
 // ---------------------------------------------------------------- adapters
 // Counted pops (kCounted): in a level-synchronous traversal a level's input

@@ -296,9 +296,14 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
   const uint32_t nblocks = d.lane_blocks[lane];
   const uint32_t tid = (blockIdx.x - first) * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)nblocks * blockDim.x;
+  uint64_t prev_end[2] = {kNoHead, kNoHead};  // per frontier: its end when last drained
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
+    // Head of the frontier drained at the last barrier back to its end (it
+    // is this level's output; counted pops overshoot Head).
+    if constexpr (F::kQueues == 2)
+      if (L > 0 && blockIdx.x == first && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
     // Queries installed at the previous barrier: clear their depth arrays
     // (0 at the source) and visited bitmaps, and put their sources into this
     // level's output.  Their first expansion is a level later, after the
@@ -311,15 +316,19 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
       uint32_t* vis = (uint32_t*)gcq::ld_rlx(&sl->visited);
       const uint32_t nv = gcq::ld_rlx(&sl->nv), src = gcq::ld_rlx(&sl->src);
       for (uint64_t v = tid; v < nv; v += nthreads) gcq::st_rlx(&dep[v], v == src ? 0 : -1);
-      for (uint64_t w = tid; w < ((uint64_t)nv + 31) / 32; w += nthreads)
-        gcq::st_rlx(&vis[w], w == src / 32 ? 1u << (src & 31u) : 0u);
+      if (vis)  // 0 = no bitmap (the slot table's convention; the host always gives one)
+        for (uint64_t w = tid; w < ((uint64_t)nv + 31) / 32; w += nthreads)
+          gcq::st_rlx(&vis[w], w == src / 32 ? 1u << (src & 31u) : 0u);
       if (tid == 0 && !fq.out(L).push(tid, (s << d.bd.vbits) | src)) gcq::or_rlx(&d.c->err, 1u);
     }
     // Slot fields, depths and bitmaps are read through the caches: they only
     // change at a barrier (admission) or, for a query not yet expanding, in
     // the level after it, and every block drops its CU's and XCD's cached
     // lines after each barrier.
-    (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage);
+    uint64_t end = 0;
+    (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage,
+                               F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
+    prev_end[L & 1] = end;
     if (!grid_barrier(&lc->bar, nblocks, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
           admit_bfs(d, lane, L);
