@@ -331,31 +331,39 @@ def secondary(g, with_cpu):
                           "launches": "5 warmup + 20 timed", "capacity": 1 << 20, "queues": sweep}
     _progress("queue sweep " + ", ".join(f"{k} {v['split25_65536']}" for k, v in sweep.items()))
 
-    # --- plain BFS on the lattice
+    # --- plain BFS on the lattice: every frontier queue, persistent mode
+    # (3 warmup + 10 timed runs each); the line reports the fastest
     with bfs.Graph.lattice(4900, 4900, 0.75, 0.01, 7) as gr:
         nv, ne = gr.info()
-        for _ in range(3):
-            gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=bfs.PERSISTENT, depths=False)
-        ms = []
-        for _ in range(10):
-            _, res = gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=bfs.PERSISTENT, depths=False)
-            ms.append(res["kernel_ms"])
-        med = float(np.median(ms))
+        per_q = {}
+        for qn in ("glfq", "gwfq", "wfq", "sfq"):
+            kind = g.KINDS[qn]
+            for _ in range(3):
+                gr.bfs(0, kind=kind, threads=8192, block=256, mode=bfs.PERSISTENT, depths=False)
+            ms = []
+            for _ in range(10):
+                _, res = gr.bfs(0, kind=kind, threads=8192, block=256, mode=bfs.PERSISTENT, depths=False)
+                ms.append(res["kernel_ms"])
+            per_q[qn] = float(np.median(ms))
+        best = min(per_q, key=per_q.get)
+        med = per_q[best]
         alg_bytes = 28 * res["reached"] + 8 * res["edges_scanned"]
-        o = {"workload": "lattice 4900x4900, keep 0.75, diag 0.01, seed 7, src 0, G-LFQ, 8192 threads, persistent",
+        o = {"workload": f"lattice 4900x4900, keep 0.75, diag 0.01, seed 7, src 0, 8192 threads, persistent, "
+                         f"{QUEUE_NAMES[g.KINDS[best]]} frontier (fastest of the four)",
              "vertices": nv, "edges": ne, "levels": res["levels"], "reached": res["reached"],
-             "edges_scanned": res["edges_scanned"], "kernel_ms_median": round(med, 3),
+             "edges_scanned": res["edges_scanned"], "queue": best, "kernel_ms_median": round(med, 3),
              "mteps": round(res["edges_scanned"] / med / 1e3, 1),
+             "per_queue_ms": {k: round(v, 3) for k, v in per_q.items()},
              "roofline": {"bound": "hbm", "achieved": round(alg_bytes / med / 1e6, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(alg_bytes / med / 1e6 / HBM_PEAK_GBS, 5),
                           "bytes_per_launch": alg_bytes}}
-        # the same traversal in the other one-launch modes (1 warmup + 5 timed)
+        # the same traversal in the other one-launch modes, same queue (1 warmup + 5 timed)
         o["other_modes"] = {}
         for mname, mode in (("bitmap", bfs.BITMAP), ("async", bfs.ASYNC)):
-            gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=mode, depths=False)
+            gr.bfs(0, kind=g.KINDS[best], threads=8192, block=256, mode=mode, depths=False)
             mm = []
             for _ in range(5):
-                _, r2 = gr.bfs(0, kind=g.GLFQ, threads=8192, block=256, mode=mode, depths=False)
+                _, r2 = gr.bfs(0, kind=g.KINDS[best], threads=8192, block=256, mode=mode, depths=False)
                 mm.append(r2["kernel_ms"])
             m2 = float(np.median(mm))
             o["other_modes"][mname] = {"kernel_ms_median": round(m2, 3),
@@ -416,7 +424,7 @@ def secondary(g, with_cpu):
         "roofline": {"bound": "hbm", "achieved": round(alg / bms / 1e6, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / bms / 1e6 / HBM_PEAK_GBS, 5), "bytes_per_launch": alg},
         "config": {k: getattr(cfg, k) for k in ("threads", "block", "pop_width", "flush_at", "capacity_log2",
-                                                 "use_graph")},
+                                                 "use_graph", "multi_source")},
         "autotune": {"trials": len(trials), "seconds": round(tune_s, 1)},
         "replayed_graph": bool(res["graph_replayed"])}
     for gr in graphs:

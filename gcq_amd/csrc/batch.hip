@@ -82,6 +82,48 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
   if (tid == 0) ctl->levels = L + 1;
 }
 
+// Multi-source groups (bfs_batch.hpp): the same level loop over group items.
+template <class F>
+__global__ void k_batch_ms(Frontiers<F> fq, MsDev d, Ctl* ctl) {
+  __shared__ uint32_t stage[16][128];
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t* st = stage[(threadIdx.x / 64) & 15];
+  uint64_t prev_end[2] = {kNoHead, kNoHead};
+  int32_t L = 0;
+  for (;; ++L) {
+    if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
+    if constexpr (F::kQueues == 2)
+      if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
+    uint64_t end = 0;
+    const uint32_t dw = wave_sum32(ms_expand<F>(fq.in(L), fq.out(L), d, L, tid, st,
+                                                F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end));
+    prev_end[L & 1] = end;
+    if (lane_id() == 0 && dw) gcq::faa_rlx(&ctl->next[L % 3], dw);
+    if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
+          if (F::kQueues == 3) fq.idle(L).rewind();
+        }))
+      break;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint32_t n = gcq::ld_rlx(&ctl->next[L % 3]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) gcq::st_rlx(&ctl->next[(L + 2) % 3], 0u);
+    if (n == 0) break;
+  }
+  if (tid == 0) ctl->levels = L + 1;
+}
+
+// Level-0 mask of each group: query i sets its bit at its source, and the
+// fetch-or that makes the word non-zero queues (group, source).
+template <class F>
+__global__ void k_seed_ms(F q, MsDev d, const uint32_t* qinfo, uint32_t nq) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t i = tid; i < nq; i += gridDim.x * blockDim.x) {
+    const uint32_t g = qinfo[3 * i], bit = qinfo[3 * i + 1], src = qinfo[3 * i + 2];
+    uint64_t* in0 = (uint64_t*)d.groups[g].nxt[1];  // read at level 0 as nxt[(0 + 1) & 1]
+    if (gcq::or_rlx(&in0[src], 1ull << bit) == 0ull && !q.push(tid, (g << d.vbits) | src)) gcq::or_rlx(d.err, 1u);
+  }
+}
+
 // Level-0 frontier: one item per query.  The grid has at most as many
 // threads as the traversal (queue thread ids must stay below that).
 template <class F>
@@ -94,26 +136,34 @@ __global__ void k_seed(F q, Dev d) {
   }
 }
 
-// Per query: reached vertices and the degree sum over them (the reference's
+// Per query: reached vertices, the degree sum over them (the reference's
 // edges_scanned: every reached vertex is dequeued once and its adjacency
-// scanned, bfs_bench.cpp:372-377).
+// scanned, bfs_bench.cpp:372-377) and its levels (largest depth + 1, as the
+// reference counts them, bfs_bench.cpp:622-631).
 __global__ void k_stats(Dev d, uint64_t* stats) {
   const uint32_t s = blockIdx.y;
   const Slot& sl = d.slots[s];
   const int32_t* off = (const int32_t*)sl.off;
   const int32_t* dep = (const int32_t*)sl.depth;
-  uint64_t r = 0, e = 0;
+  uint64_t r = 0, e = 0, m = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < sl.nv; v += gridDim.x * blockDim.x) {
-    if (dep[v] >= 0) {
+    const int32_t dv = dep[v];
+    if (dv >= 0) {
       ++r;
       e += (uint64_t)(off[v + 1] - off[v]);
+      m = (uint64_t)dv + 1 > m ? (uint64_t)dv + 1 : m;
     }
   }
   r = gcq::wave_sum(r);
   e = gcq::wave_sum(e);
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t t = shfl64(m, (int)(lane_id() ^ (uint32_t)o));
+    m = t > m ? t : m;
+  }
   if (lane_id() == 0) {
-    if (r) gcq::faa_rlx(&stats[2 * s], r);
-    if (e) gcq::faa_rlx(&stats[2 * s + 1], e);
+    if (r) gcq::faa_rlx(&stats[3 * s], r);
+    if (e) gcq::faa_rlx(&stats[3 * s + 1], e);
+    if (m) gcq::max_rlx(&stats[3 * s + 2], m);
   }
 }
 
@@ -144,9 +194,31 @@ struct gcq_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> key;
   std::vector<uint64_t> depth_at;  // per slot offset into depth
+  // multi-source groups (gcq_batch_config.multi_source)
+  bk::Group* ms_groups = nullptr;   // device, ms_group_cap entries
+  uint64_t* ms_dtab = nullptr;      // device, 64 depth-array pointers per group
+  uint32_t* ms_qinfo = nullptr;     // device, (group, bit, source) per query
+  uint32_t ms_group_cap = 0, ms_q_cap = 0;
+  uint64_t* ms_bits = nullptr;      // device: seen, nxt[0], nxt[1] of every group
+  uint64_t ms_bits_cap = 0, ms_bits_used = 0;  // words
+  uint32_t ms_ngroups = 0;
+  std::vector<bk::Group> h_groups;  // host staging (kept alive across the copies)
+  std::vector<uint64_t> h_dtab;
+  std::vector<uint32_t> h_qinfo;
 };
 
 namespace {
+
+void free_ms(gcq_batch* b) {
+  for (void* p : {(void*)b->ms_groups, (void*)b->ms_dtab, (void*)b->ms_qinfo, (void*)b->ms_bits})
+    if (p) (void)hipFree(p);
+  b->ms_groups = nullptr;
+  b->ms_dtab = nullptr;
+  b->ms_qinfo = nullptr;
+  b->ms_bits = nullptr;
+  b->ms_group_cap = b->ms_q_cap = 0;
+  b->ms_bits_cap = 0;
+}
 
 void free_slots(gcq_batch* b) {
   for (void* p : {(void*)b->d_slots, (void*)b->alive, (void*)b->done, (void*)b->stats})
@@ -191,10 +263,10 @@ int ensure_buffers(gcq_batch* b, uint32_t nq, uint64_t total_v, uint64_t visited
     GCQ_HIP(hipMalloc(&b->d_slots, sizeof(bk::Slot) * nq));
     GCQ_HIP(hipMalloc(&b->alive, 4ull * bk::kAliveStride * nq));
     GCQ_HIP(hipMalloc(&b->done, 4ull * nq));
-    GCQ_HIP(hipMalloc(&b->stats, 16ull * nq));
+    GCQ_HIP(hipMalloc(&b->stats, 24ull * nq));
     GCQ_HIP(hipHostMalloc((void**)&b->h_slots, sizeof(bk::Slot) * nq));
     GCQ_HIP(hipHostMalloc((void**)&b->h_done, 4ull * nq));
-    GCQ_HIP(hipHostMalloc((void**)&b->h_stats, 16ull * nq));
+    GCQ_HIP(hipHostMalloc((void**)&b->h_stats, 24ull * nq));
     b->slot_cap = nq;
   }
   if (!b->ctl) {
@@ -220,14 +292,23 @@ int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq, const bk::Dev& d, 
                      (uint64_t*)b->visited, b->visited_used / 2);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(2ull * bk::kAliveStride * nq)), dim3(256), 0, s,
                      (uint64_t*)b->alive, (uint64_t)bk::kAliveStride * nq / 2);
-  hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(2ull * nq)), dim3(256), 0, s, b->stats, 2ull * nq);
+  hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(3ull * nq)), dim3(256), 0, s, b->stats, 3ull * nq);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, s, (uint64_t*)b->ctl,
                      (uint64_t)(sizeof(bk::Ctl) / 8));
   hipLaunchKernelGGL(k_fill_done, dim3((nq + 255) / 256), dim3(256), 0, s, b->done, nq);
   b->fs.reset(s);
   const uint32_t seed_blocks = std::min((nq + 63) / 64, cfg->threads / 64);
-  hipLaunchKernelGGL(bk::k_seed<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], d);
-  hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d, b->ctl);
+  if (cfg->multi_source) {
+    hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(b->ms_bits_used)), dim3(256), 0, s, b->ms_bits,
+                       b->ms_bits_used);
+    const bk::MsDev md{b->ms_groups, &b->ctl->err, b->ms_ngroups, b->vbits, cfg->pop_width, cfg->flush_at};
+    hipLaunchKernelGGL(bk::k_seed_ms<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], md, (const uint32_t*)b->ms_qinfo,
+                       nq);
+    hipLaunchKernelGGL(bk::k_batch_ms<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, md, b->ctl);
+  } else {
+    hipLaunchKernelGGL(bk::k_seed<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], d);
+    hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d, b->ctl);
+  }
   const uint32_t bx = std::max<uint32_t>(1, std::min<uint32_t>(64, (max_nv + 255) / 256));
   hipLaunchKernelGGL(bk::k_stats, dim3(bx, nq), dim3(256), 0, s, d, b->stats);
   GCQ_HIP(hipGetLastError());
@@ -241,6 +322,73 @@ int check_config(const gcq_batch_config* c) {
   GCQ_REQUIRE(c->pop_width >= 1 && c->pop_width <= 64, "pop_width must be in [1,64]");
   GCQ_REQUIRE(c->flush_at >= 1 && c->flush_at <= 64, "flush_at must be in [1,64]");
   GCQ_REQUIRE(c->capacity_log2 <= 31, "capacity_log2 must be <= 31");
+  GCQ_REQUIRE(c->multi_source <= 1, "multi_source must be 0 or 1");
+  return GCQ_OK;
+}
+
+// Multi-source groups: queries sorted by graph (stably), up to 64 per group;
+// each group gets seen / nxt[0] / nxt[1] words for its graph's vertices.
+int build_groups(gcq_batch* b, int32_t nq, const int32_t* gids, const uint32_t* srcs,
+                 const std::vector<uint64_t>& depth_at, uint64_t* group_v) {
+  std::vector<int32_t> order(nq);
+  for (int32_t i = 0; i < nq; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return gids[x] < gids[y]; });
+  b->h_groups.clear();
+  b->h_dtab.clear();
+  b->h_qinfo.assign(3ull * nq, 0u);
+  uint64_t words = 0;
+  *group_v = 0;
+  for (int32_t k = 0; k < nq;) {
+    const int32_t gi = gids[order[k]];
+    const gcq_graph* gr = b->graphs[gi];
+    const uint64_t nv = (uint64_t)gr->host.nv;
+    bk::Group G{};
+    G.off = (uint64_t)gr->d_off;
+    G.idx = (uint64_t)gr->d_idx;
+    G.nv = (uint32_t)nv;
+    G.seen = words;  // offsets for now: turned into addresses once ms_bits exists
+    G.nxt[0] = words + nv;
+    G.nxt[1] = words + 2 * nv;
+    words += 3 * nv;
+    *group_v += nv;
+    const uint32_t g = (uint32_t)b->h_groups.size();
+    G.depth = 64ull * g;  // index into ms_dtab, likewise
+    uint32_t m = 0;
+    for (; k < nq && gids[order[k]] == gi && m < 64; ++k, ++m) {
+      const int32_t q = order[k];
+      b->h_qinfo[3 * q] = g;
+      b->h_qinfo[3 * q + 1] = m;
+      b->h_qinfo[3 * q + 2] = srcs[q];
+      b->h_dtab.push_back((uint64_t)(b->depth + depth_at[q]));
+    }
+    while (b->h_dtab.size() % 64) b->h_dtab.push_back(0);
+    G.nmem = m;
+    b->h_groups.push_back(G);
+  }
+  const uint32_t ng = (uint32_t)b->h_groups.size();
+  if (ng > b->ms_group_cap || (uint32_t)nq > b->ms_q_cap || words > b->ms_bits_cap) {
+    drop_graph(b);
+    free_ms(b);
+    GCQ_HIP(hipMalloc(&b->ms_groups, sizeof(bk::Group) * ng));
+    GCQ_HIP(hipMalloc(&b->ms_dtab, 8ull * 64 * ng));
+    GCQ_HIP(hipMalloc(&b->ms_qinfo, 12ull * nq));
+    GCQ_HIP(hipMalloc(&b->ms_bits, 8ull * words));
+    b->ms_group_cap = ng;
+    b->ms_q_cap = (uint32_t)nq;
+    b->ms_bits_cap = words;
+  }
+  for (bk::Group& G : b->h_groups) {
+    G.seen = (uint64_t)(b->ms_bits + G.seen);
+    G.nxt[0] = (uint64_t)(b->ms_bits + G.nxt[0]);
+    G.nxt[1] = (uint64_t)(b->ms_bits + G.nxt[1]);
+    G.depth = (uint64_t)(b->ms_dtab + G.depth);
+  }
+  b->ms_ngroups = ng;
+  b->ms_bits_used = words;
+  GCQ_HIP(hipMemcpyAsync(b->ms_groups, b->h_groups.data(), sizeof(bk::Group) * ng, hipMemcpyHostToDevice, b->stream));
+  GCQ_HIP(hipMemcpyAsync(b->ms_dtab, b->h_dtab.data(), 8ull * b->h_dtab.size(), hipMemcpyHostToDevice, b->stream));
+  GCQ_HIP(hipMemcpyAsync(b->ms_qinfo, b->h_qinfo.data(), 12ull * nq, hipMemcpyHostToDevice, b->stream));
+  GCQ_HIP(hipStreamSynchronize(b->stream));  // the staging vectors may change on the next call
   return GCQ_OK;
 }
 
@@ -270,7 +418,16 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
   rc = ensure_buffers(b, (uint32_t)nq, total_v, total_w);
   b->visited_used = total_w;
   if (rc) return rc;
-  const uint64_t need = total_v;
+  uint64_t need = total_v;  // one item per (query, vertex) at most per level
+  if (cfg->multi_source) {
+    std::vector<uint32_t> src_ok(nq);
+    for (int32_t i = 0; i < nq; ++i) {
+      const int32_t nv = b->graphs[gids[i]]->host.nv;
+      src_ok[i] = (srcs[i] >= 0 && srcs[i] < nv) ? (uint32_t)srcs[i] : 0u;  // bfs_bench.cpp:555
+    }
+    rc = build_groups(b, nq, gids, src_ok.data(), at, &need);  // one item per (group, vertex)
+    if (rc) return rc;
+  }
   uint64_t cap_items = cfg->capacity_log2 ? (1ull << cfg->capacity_log2) : need;
   GCQ_REQUIRE(cap_items >= need, "capacity 2^%u is below the %llu items a level may hold", cfg->capacity_log2,
               (unsigned long long)need);
@@ -299,7 +456,10 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
   GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   rc = b->fs.dispatch([&](const auto& fq) {
     using F = std::decay_t<decltype(fq.q[0])>;
-    GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bk::k_batch<F>, cfg->block, 0));
+    if (cfg->multi_source)
+      GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bk::k_batch_ms<F>, cfg->block, 0));
+    else
+      GCQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bk::k_batch<F>, cfg->block, 0));
     return GCQ_OK;
   });
   if (rc) return rc;
@@ -315,7 +475,8 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
     const std::vector<uint64_t> key = {(uint64_t)cfg->kind, cfg->threads, cfg->block, cfg->pop_width, cfg->flush_at,
                                        cfg->capacity_log2, (uint64_t)nq, total_v, max_nv,
                                        (uint64_t)b->fs.q[0], (uint64_t)b->depth, (uint64_t)b->d_slots,
-                                       (uint64_t)b->visited, total_w};
+                                       (uint64_t)b->visited, total_w, cfg->multi_source,
+                                       (uint64_t)b->ms_bits, b->ms_bits_used, b->ms_ngroups};
     if (key != b->key || !b->exec) {
       drop_graph(b);
       GCQ_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -340,7 +501,7 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
   }
   GCQ_HIP(hipEventRecord(e1, b->stream));
   GCQ_HIP(hipMemcpyAsync(b->h_done, b->done, 4ull * nq, hipMemcpyDeviceToHost, b->stream));
-  GCQ_HIP(hipMemcpyAsync(b->h_stats, b->stats, 16ull * nq, hipMemcpyDeviceToHost, b->stream));
+  GCQ_HIP(hipMemcpyAsync(b->h_stats, b->stats, 24ull * nq, hipMemcpyDeviceToHost, b->stream));
   GCQ_HIP(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(bk::Ctl), hipMemcpyDeviceToHost, b->stream));
   GCQ_HIP(hipStreamSynchronize(b->stream));
   res->kernel_ms = gcqh::ms_between(e0, e1);
@@ -348,12 +509,12 @@ int run_batch(gcq_batch* b, const gcq_batch_config* cfg, int32_t nq, const int32
   (void)hipEventDestroy(e1);
   res->levels = b->h_ctl->levels;
   for (int32_t i = 0; i < nq; ++i) {
-    res->edges_scanned += b->h_stats[2 * i + 1];
+    res->edges_scanned += b->h_stats[3 * i + 1];
     if (per_query) {
-      per_query[i].reached = b->h_stats[2 * i];
-      per_query[i].edges_scanned = b->h_stats[2 * i + 1];
-      per_query[i].levels = b->h_done[i];
-      per_query[i].status = b->h_done[i] > 0 ? 0 : -1;
+      per_query[i].reached = b->h_stats[3 * i];
+      per_query[i].edges_scanned = b->h_stats[3 * i + 1];
+      per_query[i].levels = (int32_t)b->h_stats[3 * i + 2];  // largest depth + 1
+      per_query[i].status = per_query[i].levels > 0 ? 0 : -1;
     }
   }
   res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -468,12 +629,20 @@ extern "C" int gcq_batch_autotune(gcq_batch* b, int32_t kind, int32_t nq, const 
   double best_ms = 1e300;
   int rc = measure(cur, &best_ms);
   if (rc) return rc;
-  // Coordinate descent, two sweeps over the knobs.
+  // Coordinate descent, two sweeps over the knobs: the traversal form
+  // (multi-source groups or one item per query) first, then the grid.
   for (int sweep = 0; sweep < 2; ++sweep) {
-    for (int knob = 0; knob < 5; ++knob) {
+    for (int knob : {5, 0, 1, 2, 3, 4}) {
       std::vector<gcq_batch_config> cand;
       gcq_batch_config c = cur;
       switch (knob) {
+        case 5:
+          for (uint32_t ms : {0u, 1u}) {
+            c = cur;
+            c.multi_source = ms;
+            cand.push_back(c);
+          }
+          break;
         case 0:
           for (uint32_t blk : {128u, 256u, 512u}) {
             c = cur;
@@ -534,6 +703,7 @@ extern "C" int gcq_batch_destroy(gcq_batch* b) {
   drop_graph(b);
   b->fs.release();
   free_slots(b);
+  free_ms(b);
   if (b->depth) (void)hipFree(b->depth);
   if (b->visited) (void)hipFree(b->visited);
   if (b->ctl) (void)hipFree(b->ctl);

@@ -180,3 +180,153 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
 }
 
 }  // namespace bk
+
+// ---------------------------------------------------------------- multi-source groups
+namespace bk {
+
+// Bit-parallel multi-source BFS (the MS-BFS of Then et al., "The More the
+// Merrier: Efficient Multi-Source Graph Traversal", PVLDB 8(4), 2014) on the
+// queue-driven, level-synchronous frontier.  Up to 64 queries on one graph
+// form a group; a work item is (group, vertex), and the vertex's next-level
+// mask holds the members that reach it at that level, so an edge is scanned
+// once per level for the whole group instead of once per query.  Per vertex
+// v of the group's graph:
+//   seen[v]    members that have reached v, written only by the lane that
+//              settles v (a vertex is queued at most once per level);
+//   nxt[p][v]  members arriving at the next level of parity p, gathered by
+//              fetch-or; the fetch-or that makes it non-zero queues (g, v).
+// Settling v at level L: fresh = nxt[(L+1)&1][v] & ~seen[v] (the word is
+// cleared for the level after next), each member in fresh gets depth L at
+// v, and v's neighbours w receive fresh & ~seen[w] (seen[w] may be stale: a
+// bit w already has is dropped again when w settles).  A bit moves one edge
+// per level and only its first arrival at v is kept, so every member's depth
+// array equals a plain BFS's bit for bit.
+struct Group {
+  uint64_t off, idx;  // CSR of the group's graph
+  uint64_t seen;      // uint64_t[nv]
+  uint64_t nxt[2];    // uint64_t[nv] each
+  uint64_t depth;     // uint64_t[64]: each member's depth array (int32_t[nv])
+  uint32_t nv, nmem;
+  uint64_t pad;
+};
+static_assert(sizeof(Group) == 64, "group");
+
+struct MsDev {
+  const Group* groups;
+  uint32_t* err;  // bit 0: a frontier enqueue failed
+  uint32_t ngroups, vbits, pop_width, flush_at;
+};
+
+// As expand(), for group items: counted pops, grid-position tickets on
+// one-round levels, wave-cooperative scan of the settled vertices' edges.
+template <class F>
+__device__ uint32_t ms_expand(const F& qin, const F& qout, const MsDev& d, int32_t L, uint32_t tid, uint32_t* stage,
+                              uint64_t h0, uint64_t grid_waves, uint64_t* end_out) {
+  const uint32_t lane = lane_id();
+  const uint32_t vmask = (1u << d.vbits) - 1u;
+  uint32_t sc = 0, disc_n = 0;
+  const bool pops = lane < d.pop_width;
+  const uint64_t tix = h0 + (uint64_t)lane * grid_waves + tid / 64;
+  uint64_t first_word = 0;
+  if constexpr (F::kHeadFromDrain) {
+    if (h0 != kNoHead && pops) first_word = qin.peek(tix);
+  }
+  const uint64_t end = qin.level_end();
+  *end_out = end;
+  const bool one_round = F::kHeadFromDrain && h0 != kNoHead && end >= h0 && end - h0 <= grid_waves * d.pop_width;
+  for (;;) {
+    uint32_t item = 0;
+    int st = -1;
+    if (pops) {
+      if constexpr (F::kHeadFromDrain) {
+        st = one_round ? qin.take(tid, tix, end, first_word, &item) : qin.pop_counted(tid, end, &item);
+      } else {
+        st = qin.pop_counted(tid, end, &item);
+      }
+    }
+    if (__ballot(st >= 0) == 0) break;  // every popping lane drew a ticket past the level's end
+    uint32_t g = 0;
+    int32_t beg = 0, deg = 0;
+    uint64_t idxp = 0, seenp = 0, outp = 0, fresh = 0;
+    if (st > 0) {
+      g = item >> d.vbits;
+      const uint32_t v = item & vmask;
+      const Group* G = &d.groups[g];  // fixed for the launch: read through the caches
+      const int32_t* off = (const int32_t*)gcq::ld_cached(&G->off);
+      idxp = gcq::ld_cached(&G->idx);
+      seenp = gcq::ld_cached(&G->seen);
+      outp = gcq::ld_cached(&G->nxt[L & 1]);
+      uint64_t* const in = (uint64_t*)gcq::ld_cached(&G->nxt[(L + 1) & 1]) + v;
+      uint64_t* const sp = (uint64_t*)seenp + v;
+      const uint64_t arrived = gcq::ld_rlx(in);
+      const uint64_t had = gcq::ld_rlx(sp);
+      gcq::st_rlx(in, 0ull);  // this word gathers the level after next
+      fresh = arrived & ~had;
+      if (fresh) {
+        gcq::st_rlx(sp, had | fresh);
+        const uint64_t* dtab = (const uint64_t*)gcq::ld_cached(&G->depth);
+        for (uint64_t bits = fresh; bits; bits &= bits - 1)
+          ((int32_t*)gcq::ld_cached(&dtab[__builtin_ctzll(bits)]))[v] = L;
+        beg = off[v];
+        deg = off[v + 1] - beg;
+      }
+    }
+    int32_t incl = deg;  // inclusive prefix of degrees across the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t t = __shfl_up(incl, o, 64);
+      if ((int)lane >= o) incl += t;
+    }
+    const int32_t excl = incl - deg;
+    const int32_t total = __shfl(incl, 63, 64);
+    auto owner = [&](int32_t base) {  // first lane whose inclusive prefix exceeds this lane's edge
+      const int32_t e = base + (int32_t)lane;
+      const int32_t ec = e < total ? e : total - 1;
+      int lo = 0, hi = 63;
+      for (int s = 0; s < 6; ++s) {
+        const int mid = (lo + hi) >> 1;
+        if (__shfl(incl, mid, 64) > ec) hi = mid;
+        else lo = mid + 1;
+      }
+      return lo;
+    };
+    auto neighbour = [&](int32_t base, int lo) -> uint32_t {  // this lane's edge target, 0 past the end
+      const int32_t e = base + (int32_t)lane;
+      const int32_t obeg = __shfl(beg, lo, 64);
+      const int32_t oexcl = __shfl(excl, lo, 64);
+      const uint64_t oidx = shfl64(idxp, lo);
+      return e < total ? (uint32_t)((const int32_t*)oidx)[obeg + (e - oexcl)] : 0u;
+    };
+    int lo_next = total > 0 ? owner(0) : 0;
+    uint32_t w_next = total > 0 ? neighbour(0, lo_next) : 0u;
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t e = base + (int32_t)lane;
+      const int lo = lo_next;
+      const uint32_t w_here = w_next;
+      if (base + 64 < total) {
+        lo_next = owner(base + 64);
+        w_next = neighbour(base + 64, lo_next);
+      }
+      const uint32_t og = __shfl(g, lo, 64);
+      const uint64_t ofresh = shfl64(fresh, lo);
+      const uint64_t oseen = shfl64(seenp, lo);
+      const uint64_t oout = shfl64(outp, lo);
+      bool disc = false;
+      uint32_t w = 0;
+      if (e < total) {
+        w = w_here;
+        const uint64_t send = ofresh & ~gcq::ld_cached((const uint64_t*)oseen + w);
+        if (send) disc = gcq::or_rlx((uint64_t*)oout + w, send) == 0ull;
+      }
+      const uint64_t dm = __ballot(disc);
+      if (disc) stage[sc + gcq::rank_in(dm)] = (og << d.vbits) | w;
+      sc += (uint32_t)__builtin_popcountll(dm);
+      disc_n += disc ? 1u : 0u;
+      while (sc >= d.flush_at) flush(qout, tid, stage, sc, sc < 64u ? sc : 64u, d.err);
+    }
+    if (one_round) break;  // every ticket of the level was handed out in this round
+  }
+  if (sc) flush(qout, tid, stage, sc, sc, d.err);
+  return disc_n;
+}
+
+}  // namespace bk

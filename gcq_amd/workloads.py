@@ -15,7 +15,7 @@ class BfsResult(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("threads", C.c_uint32), ("block", C.c_uint32), ("pop_width", C.c_uint32),
                 ("flush_at", C.c_uint32), ("capacity_log2", C.c_uint32), ("use_graph", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("multi_source", C.c_uint32)]
 
 
 class QueryResult(C.Structure):

@@ -280,7 +280,9 @@ typedef struct gcq_batch_config {
   uint32_t flush_at;      /* staged discoveries per wave before an aggregated enqueue (8..64) */
   uint32_t capacity_log2; /* frontier ring size; 0 = smallest power of two >= sum of query V */
   uint32_t use_graph;     /* capture the launch sequence in a hipGraph and replay it */
-  uint32_t pad;
+  uint32_t multi_source;  /* 1: queries on one graph run in groups of up to 64 as one bit-parallel traversal
+                             (a (group, vertex) item per vertex and level, with a 64-bit mask of the member
+                             queries that reach it); same depths bit for bit.  0: one item per (query, vertex) */
 } gcq_batch_config;
 typedef struct gcq_query_result {
   uint64_t reached;
