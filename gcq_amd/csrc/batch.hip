@@ -345,7 +345,8 @@ int build_groups(gcq_batch* b, int32_t nq, const int32_t* gids, const uint32_t* 
     bk::Group G{};
     G.off = (uint64_t)gr->d_off;
     G.idx = (uint64_t)gr->d_idx;
-    G.nv = (uint32_t)nv;
+    G.start = 0;  // every member starts at level 0
+    G.fresh = 0;
     G.seen = words;  // offsets for now: turned into addresses once ms_bits exists
     G.nxt[0] = words + nv;
     G.nxt[1] = words + 2 * nv;
@@ -362,7 +363,6 @@ int build_groups(gcq_batch* b, int32_t nq, const int32_t* gids, const uint32_t* 
       b->h_dtab.push_back((uint64_t)(b->depth + depth_at[q]));
     }
     while (b->h_dtab.size() % 64) b->h_dtab.push_back(0);
-    G.nmem = m;
     b->h_groups.push_back(G);
   }
   const uint32_t ng = (uint32_t)b->h_groups.size();

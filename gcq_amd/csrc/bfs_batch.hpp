@@ -196,18 +196,27 @@ namespace bk {
 //   nxt[p][v]  members arriving at the next level of parity p, gathered by
 //              fetch-or; the fetch-or that makes it non-zero queues (g, v).
 // Settling v at level L: fresh = nxt[(L+1)&1][v] & ~seen[v] (the word is
-// cleared for the level after next), each member in fresh gets depth L at
-// v, and v's neighbours w receive fresh & ~seen[w] (seen[w] may be stale: a
-// bit w already has is dropped again when w settles).  A bit moves one edge
-// per level and only its first arrival at v is kept, so every member's depth
-// array equals a plain BFS's bit for bit.
+// cleared for the level after next), each member in fresh gets depth
+// L - start at v, and v's neighbours w receive fresh & ~seen[w] (seen[w] may
+// be stale: a bit w already has is dropped again when w settles).  A bit
+// moves one edge per level and only its first arrival at v is kept, so every
+// member's depth array equals a plain BFS's bit for bit.
+//
+// kLive (the server): members join and leave while the kernel runs.  A
+// member's bit is cleared from seen[] (atomic and, during the level after
+// its admission) before it is seeded, so settling takes its fresh bits with
+// one fetch-or on seen[v] (a load and a store could write a cleared bit
+// back), and cached reads of seen[] are current from the next barrier on
+// (every block drops its cached lines after a barrier).  Each level's fresh
+// members are gathered per group (one fetch-or per wave and group) so that a
+// member with nothing fresh at a level is known to be finished.
 struct Group {
   uint64_t off, idx;  // CSR of the group's graph
   uint64_t seen;      // uint64_t[nv]
   uint64_t nxt[2];    // uint64_t[nv] each
   uint64_t depth;     // uint64_t[64]: each member's depth array (int32_t[nv])
-  uint32_t nv, nmem;
-  uint64_t pad;
+  uint64_t start;     // int32_t[64]: level at which each member's source settles (0 = all members at 0)
+  uint64_t fresh;     // kLive: uint64_t[2], the members settled at a level of each parity
 };
 static_assert(sizeof(Group) == 64, "group");
 
@@ -219,7 +228,7 @@ struct MsDev {
 
 // As expand(), for group items: counted pops, grid-position tickets on
 // one-round levels, wave-cooperative scan of the settled vertices' edges.
-template <class F>
+template <class F, bool kLive = false>
 __device__ uint32_t ms_expand(const F& qin, const F& qout, const MsDev& d, int32_t L, uint32_t tid, uint32_t* stage,
                               uint64_t h0, uint64_t grid_waves, uint64_t* end_out) {
   const uint32_t lane = lane_id();
@@ -259,16 +268,36 @@ __device__ uint32_t ms_expand(const F& qin, const F& qout, const MsDev& d, int32
       uint64_t* const in = (uint64_t*)gcq::ld_cached(&G->nxt[(L + 1) & 1]) + v;
       uint64_t* const sp = (uint64_t*)seenp + v;
       const uint64_t arrived = gcq::ld_rlx(in);
-      const uint64_t had = gcq::ld_rlx(sp);
       gcq::st_rlx(in, 0ull);  // this word gathers the level after next
-      fresh = arrived & ~had;
+      if constexpr (kLive) {
+        fresh = arrived ? arrived & ~gcq::or_rlx(sp, arrived) : 0ull;
+      } else {
+        const uint64_t had = gcq::ld_rlx(sp);
+        fresh = arrived & ~had;
+        if (fresh) gcq::st_rlx(sp, had | fresh);
+      }
       if (fresh) {
-        gcq::st_rlx(sp, had | fresh);
         const uint64_t* dtab = (const uint64_t*)gcq::ld_cached(&G->depth);
-        for (uint64_t bits = fresh; bits; bits &= bits - 1)
-          ((int32_t*)gcq::ld_cached(&dtab[__builtin_ctzll(bits)]))[v] = L;
+        const int32_t* stab = (const int32_t*)gcq::ld_cached(&G->start);
+        for (uint64_t bits = fresh; bits; bits &= bits - 1) {
+          const int b = __builtin_ctzll(bits);
+          ((int32_t*)gcq::ld_cached(&dtab[b]))[v] = stab ? L - gcq::ld_cached(&stab[b]) : L;
+        }
         beg = off[v];
         deg = off[v + 1] - beg;
+      }
+    }
+    if constexpr (kLive) {
+      // this level's fresh members, one fetch-or per group present in the wave
+      uint64_t pend = __ballot(fresh != 0ull);
+      while (pend) {
+        const int l0 = __builtin_ctzll(pend);
+        const uint32_t gs = __shfl(g, l0, 64);
+        const bool mine = fresh != 0ull && g == gs;
+        uint64_t m = mine ? fresh : 0ull;
+        for (int o = 32; o > 0; o >>= 1) m |= shfl64(m, (int)(lane ^ (uint32_t)o));
+        if ((int)lane == l0) gcq::or_rlx((uint64_t*)gcq::ld_cached(&d.groups[gs].fresh) + (L & 1), m);
+        pend &= ~__ballot(mine);
       }
     }
     int32_t incl = deg;  // inclusive prefix of degrees across the wave

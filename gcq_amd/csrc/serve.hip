@@ -125,6 +125,18 @@ struct Ctl {
   int32_t active_rt, p7[31];
 };
 
+// Multi-source groups (gcq_server_config.multi_source): per group, its
+// members and the members settled at a level of each parity, on one line;
+// written at admission and by done_scan (one thread per group), gathered
+// into by the expansion.
+struct GroupCtl {
+  uint64_t members;   // busy bits
+  uint64_t fresh[2];  // bk::Group.fresh points here
+  uint32_t lane, nv;
+  uint64_t pad[12];
+};
+static_assert(sizeof(GroupCtl) == 128, "group ctl");
+
 struct Dev {
   bk::Dev bd;              // read side of the BFS slot table
   bk::Slot* slots;         // the same table, written at admission
@@ -143,6 +155,15 @@ struct Dev {
   int32_t W, H, max_depth;
   uint32_t njobs, bfs_blocks, max_requests, rt_workers;  // rt_workers: the RT sub-grid runs tile jobs
   uint32_t lane_blocks[kLanes];  // blocks [0, lane_blocks[0]) are lane 0, the next lane_blocks[1] lane 1
+  // multi-source groups (ms != 0): the group table, its control lines and
+  // per member (64 per group) the depth array, start level and BFS slot
+  uint32_t ms, ngroups;
+  bk::MsDev md;
+  bk::Group* groups;
+  GroupCtl* gctl;
+  uint64_t* gdepth;
+  int32_t* gstart;
+  uint32_t* gslot;
 };
 
 __device__ void publish(const Dev& d, uint64_t id, int32_t type, int32_t levels, int32_t slot, uint64_t rays) {
@@ -226,6 +247,14 @@ __device__ void admit_bfs(const Dev& d, int lane, int32_t L) {
     gcq::st_rlx(&s->nv, g.nv);
     gcq::st_rlx(&s->start, (uint32_t)(L + 2));  // cleared during L+1, source expanded at L+2
     gcq::st_rlx(&s->pad2[0], gcq::ld_rlx(&e->id));
+    if (d.ms) {  // the host chose the member: group in x0, bit in y0
+      const uint32_t gi = (uint32_t)gcq::ld_rlx(&e->x0), b = (uint32_t)gcq::ld_rlx(&e->y0);
+      gcq::st_rlx(&s->pad2[1], ((uint64_t)gi << 8) | b);
+      gcq::st_rlx(&d.gdepth[64ull * gi + b], gcq::ld_rlx(&e->buf));
+      gcq::st_rlx(&d.gstart[64ull * gi + b], L + 2);
+      gcq::st_rlx(&d.gslot[64ull * gi + b], slot);
+      gcq::st_rlx(&d.gctl[gi].members, gcq::ld_rlx(&d.gctl[gi].members) | (1ull << b));
+    }
     gcq::st_rlx(&d.bd.alive[slot * bk::kAliveStride], (uint32_t)(L + 2));
     gcq::st_rlx(&d.slot_active[slot], 1u + (uint32_t)lane);
     gcq::st_rlx(&lc->adm_slot[n], slot);
@@ -290,6 +319,33 @@ __device__ void done_scan(const Dev& d, int lane, int32_t L) {
 }
 
 // One BFS lane's sub-grid: blocks [first, first + lane_blocks[lane]).
+// Multi-source form of done_scan (the lane's first block after the barrier
+// ending its level L, one thread per group): a member with no vertex settled
+// at level L is finished; its levels are L - start.  The level's gather word
+// is cleared for level L + 2.
+__device__ void done_scan_ms(const Dev& d, int lane, int32_t L) {
+  for (uint32_t gi = threadIdx.x; gi < d.ngroups; gi += blockDim.x) {
+    GroupCtl* gc = &d.gctl[gi];
+    if (gcq::ld_rlx(&gc->lane) != (uint32_t)lane) continue;
+    const uint64_t act = gcq::ld_rlx(&gc->fresh[L & 1]);
+    gcq::st_rlx(&gc->fresh[L & 1], 0ull);
+    uint64_t mem = gcq::ld_rlx(&gc->members);
+    for (uint64_t bits = mem; bits; bits &= bits - 1) {
+      const int b = __builtin_ctzll(bits);
+      const int32_t start = gcq::ld_rlx(&d.gstart[64ull * gi + b]);
+      if (start > L || ((act >> b) & 1ull)) continue;
+      const uint32_t slot = gcq::ld_rlx(&d.gslot[64ull * gi + b]);
+      const uint64_t id = gcq::ld_rlx(&d.slots[slot].pad2[0]);
+      mem &= ~(1ull << b);
+      gcq::st_rlx(&gc->members, mem);
+      gcq::st_rlx(&d.slot_active[slot], 0u);
+      gcq::faa_rlx(&d.c->lane[lane].active, -1);
+      gcq::vm_wait();  // member and slot free, count down, before the host can reuse them
+      publish(d, id, GCQ_REQ_BFS, L - start, (int32_t)slot, 0);
+    }
+  }
+}
+
 template <class F>
 __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_t first, uint32_t* stage) {
   LaneCtl* const lc = &d.c->lane[lane];
@@ -313,21 +369,44 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
       const uint32_t s = gcq::ld_rlx(&lc->adm_slot[a]);
       const bk::Slot* sl = &d.slots[s];
       int32_t* dep = (int32_t*)gcq::ld_rlx(&sl->depth);
-      uint32_t* vis = (uint32_t*)gcq::ld_rlx(&sl->visited);
       const uint32_t nv = gcq::ld_rlx(&sl->nv), src = gcq::ld_rlx(&sl->src);
-      for (uint64_t v = tid; v < nv; v += nthreads) gcq::st_rlx(&dep[v], v == src ? 0 : -1);
-      if (vis)  // 0 = no bitmap (the slot table's convention; the host always gives one)
-        for (uint64_t w = tid; w < ((uint64_t)nv + 31) / 32; w += nthreads)
-          gcq::st_rlx(&vis[w], w == src / 32 ? 1u << (src & 31u) : 0u);
-      if (tid == 0 && !fq.out(L).push(tid, (s << d.bd.vbits) | src)) gcq::or_rlx(&d.c->err, 1u);
+      if (d.ms) {
+        // Clear the member's bit from its group's seen words (the bit's last
+        // user may have left it set; other members settle concurrently, so
+        // by atomic and), and gather the source into this level's word: it
+        // settles at level L + 1 with depth 0.
+        const uint64_t gb = gcq::ld_rlx(&sl->pad2[1]);
+        const uint32_t gi = (uint32_t)(gb >> 8);
+        const uint64_t bit = 1ull << (gb & 63u);
+        const bk::Group& G = d.groups[gi];
+        uint64_t* seen = (uint64_t*)G.seen;
+        for (uint64_t v = tid; v < nv; v += nthreads) {
+          gcq::st_rlx(&dep[v], -1);
+          if (gcq::ld_rlx(&seen[v]) & bit) __hip_atomic_fetch_and(&seen[v], ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tid == 0 && gcq::or_rlx((uint64_t*)G.nxt[L & 1] + src, bit) == 0ull &&
+            !fq.out(L).push(tid, (gi << d.bd.vbits) | src))
+          gcq::or_rlx(&d.c->err, 1u);
+      } else {
+        uint32_t* vis = (uint32_t*)gcq::ld_rlx(&sl->visited);
+        for (uint64_t v = tid; v < nv; v += nthreads) gcq::st_rlx(&dep[v], v == src ? 0 : -1);
+        if (vis)  // 0 = no bitmap (the slot table's convention; the host always gives one)
+          for (uint64_t w = tid; w < ((uint64_t)nv + 31) / 32; w += nthreads)
+            gcq::st_rlx(&vis[w], w == src / 32 ? 1u << (src & 31u) : 0u);
+        if (tid == 0 && !fq.out(L).push(tid, (s << d.bd.vbits) | src)) gcq::or_rlx(&d.c->err, 1u);
+      }
     }
     // Slot fields, depths and bitmaps are read through the caches: they only
     // change at a barrier (admission) or, for a query not yet expanding, in
     // the level after it, and every block drops its CU's and XCD's cached
     // lines after each barrier.
     uint64_t end = 0;
-    (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage,
-                               F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
+    if (d.ms)
+      (void)bk::ms_expand<F, true>(fq.in(L), fq.out(L), d.md, L, tid, stage,
+                                   F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
+    else
+      (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage,
+                                 F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
     prev_end[L & 1] = end;
     if (!grid_barrier(&lc->bar, nblocks, [&]() {
           if (F::kQueues == 3) fq.idle(L).rewind();
@@ -338,7 +417,10 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (gcq::ld_rlx(&lc->quit)) break;
-    if (blockIdx.x == first) done_scan(d, lane, L);
+    if (blockIdx.x == first) {
+      if (d.ms) done_scan_ms(d, lane, L);
+      else done_scan(d, lane, L);
+    }
   }
   if (tid == 0) lc->levels = L + 1;
 }
@@ -515,6 +597,7 @@ struct Pending {
 struct InFlight {
   gcq_request r;
   int32_t index, slot, lane;
+  int32_t group, bit;  // multi-source member, or -1
   uint64_t off1, len1, off2, len2;
   uint32_t nv;
   double admit_ms;
@@ -548,6 +631,15 @@ struct gcq_server {
   char* subq = nullptr;        // the submission queues (G-LFQ rings): BFS lane 0, BFS lane 1, RT
   uint64_t subq_off[3] = {0, 0, 0};
   sv::Req* desc = nullptr;     // request descriptors: bfs_slots BFS, then rt_slots RT
+  // multi-source groups: per graph ceil(bfs_slots / 64) groups of 64 members
+  bk::Group* groups = nullptr;
+  sv::GroupCtl* gctl = nullptr;
+  uint64_t* gdepth = nullptr;
+  int32_t* gstart = nullptr;
+  uint32_t* gslot = nullptr;
+  uint64_t* gbits = nullptr;         // seen, nxt[0], nxt[1] of every group
+  std::vector<std::vector<int32_t>> graph_groups;  // group indices of each graph
+  std::vector<uint64_t> gbusy;       // members in flight, per group (host view)
   char* arena_base = nullptr;  // in-flight request state, budget_bytes
   sv::Arena arena;
   char* keep_base = nullptr;   // kept results, keep_bytes
@@ -579,7 +671,8 @@ constexpr double kStopTimeoutMs = 30000.0;
 void release_all(gcq_server* s) {
   for (void* p : {(void*)s->slots, (void*)s->alive, (void*)s->slot_active, (void*)s->dgraphs, (void*)s->jobs,
                   (void*)s->rings, (void*)s->ctl, (void*)s->subq, (void*)s->desc, (void*)s->arena_base,
-                  (void*)s->keep_base})
+                  (void*)s->keep_base, (void*)s->groups, (void*)s->gctl, (void*)s->gdepth, (void*)s->gstart,
+                  (void*)s->gslot, (void*)s->gbits})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)s->sub, (void*)s->comp, (void*)s->stop})
     if (p) (void)hipHostFree(p);
@@ -595,8 +688,8 @@ int check_request(gcq_server* s, const gcq_request* r) {
   if (r->type == GCQ_REQ_BFS) {
     GCQ_REQUIRE(r->graph >= 0 && r->graph < (int32_t)s->graphs.size(), "bad graph %d", r->graph);
     const uint64_t nv = (uint64_t)s->graphs[r->graph]->host.nv;
-    GCQ_REQUIRE(4 * nv + vis_bytes(nv) <= s->cfg.budget_bytes, "query needs %llu bytes, above the budget",
-                (unsigned long long)(4 * nv + vis_bytes(nv)));
+    const uint64_t qbytes = 4 * nv + (s->cfg.multi_source ? 0 : vis_bytes(nv));
+    GCQ_REQUIRE(qbytes <= s->cfg.budget_bytes, "query needs %llu bytes, above the budget", (unsigned long long)qbytes);
     GCQ_REQUIRE(nv <= s->lane_cap[s->glane[r->graph]], "graph larger than its lane's frontier ring");
   } else if (r->type == GCQ_REQ_RT) {
     GCQ_REQUIRE(s->scene, "server has no scene");
@@ -625,29 +718,49 @@ int try_admit(gcq_server* s) {
       gcq_graph* g = s->graphs[r.graph];
       const uint32_t nv = (uint32_t)g->host.nv;
       const int32_t lane = s->glane[r.graph];
-      if (s->free_bfs.empty() || s->frontier_used[lane] + nv > s->lane_cap[lane]) break;
+      const bool ms = s->cfg.multi_source != 0;
+      if (s->free_bfs.empty() || (!ms && s->frontier_used[lane] + nv > s->lane_cap[lane])) break;
+      int32_t gsel = -1, bsel = -1;
+      if (ms) {  // a free member bit in one of the graph's groups
+        for (int32_t gi : s->graph_groups[r.graph])
+          if (~s->gbusy[gi]) {
+            gsel = gi;
+            bsel = __builtin_ctzll(~s->gbusy[gi]);
+            break;
+          }
+        if (gsel < 0) break;
+      }
       sv::Arena& a1 = r.keep ? s->keep : s->arena;
       if (!a1.alloc(4ull * nv, &f.off1)) {
         GCQ_REQUIRE(!r.keep, "keep_bytes exhausted by kept results");
         break;
       }
-      if (!s->arena.alloc(vis_bytes(nv), &f.off2)) {
+      const uint64_t vb = ms ? 0 : vis_bytes(nv);  // the groups' seen words replace the bitmap
+      if (vb && !s->arena.alloc(vb, &f.off2)) {
         a1.release(f.off1, 4ull * nv);
         break;
       }
       f.len1 = 4ull * nv;
-      f.len2 = vis_bytes(nv);
+      f.len2 = vb;
+      f.group = gsel;
+      f.bit = bsel;
+      if (ms) {
+        s->gbusy[gsel] |= 1ull << bsel;
+        e.x0 = gsel;
+        e.y0 = bsel;
+      }
       f.nv = nv;
       f.slot = s->free_bfs.back();
       f.lane = lane;
       s->free_bfs.pop_back();
-      s->frontier_used[lane] += nv;
+      if (!ms) s->frontier_used[lane] += nv;  // groups' items are bounded at start
       e.graph = r.graph;
       e.src = (r.src >= 0 && r.src < (int32_t)nv) ? r.src : 0;  // bfs_bench.cpp:555
       e.buf = (uint64_t)((r.keep ? s->keep_base : s->arena_base) + f.off1);
-      e.buf2 = (uint64_t)(s->arena_base + f.off2);
+      e.buf2 = vb ? (uint64_t)(s->arena_base + f.off2) : 0;
     } else {
       if (s->free_rt.empty()) break;
+      f.group = f.bit = -1;
       const uint64_t img = 12ull * r.w * r.h, rec = (uint64_t)sizeof(rtk::RayRec) * r.w * r.h;
       sv::Arena& a1 = r.keep ? s->keep : s->arena;
       if (!a1.alloc(img, &f.off1)) {
@@ -703,11 +816,12 @@ void collect(gcq_server* s) {
     c.done_ms = s->now_ms();
     if (f.r.type == GCQ_REQ_BFS) {
       s->free_bfs.push_back(f.slot);
-      s->frontier_used[f.lane] -= f.nv;
+      if (f.group >= 0) s->gbusy[f.group] &= ~(1ull << f.bit);
+      else s->frontier_used[f.lane] -= f.nv;
     } else {
       s->free_rt.push_back(f.slot);
     }
-    s->arena.release(f.off2, f.len2);  // BFS bitmap or RT ray records
+    if (f.len2) s->arena.release(f.off2, f.len2);  // BFS bitmap or RT ray records
     if (f.r.keep) s->kept[f.r.id] = sv::Kept{f.r.type, f.off1, f.len1};
     else s->arena.release(f.off1, f.len1);  // kept results stay in the keep arena until destroy
     s->ready.push_back(c);
@@ -769,6 +883,7 @@ extern "C" int gcq_server_create(gcq_graph* const* graphs, int32_t ngraphs, gcq_
               "bad pop_width / flush_at");
   GCQ_REQUIRE(cfg->max_requests >= 1 && cfg->budget_bytes >= 4096 && cfg->max_depth >= 0, "bad limits");
   GCQ_REQUIRE(cfg->small_log2 <= 30, "small_log2 must be at most 30");
+  GCQ_REQUIRE(cfg->multi_source <= 1, "multi_source must be 0 or 1");
   GCQ_REQUIRE(!cfg->small_log2 || (cfg->small_threads >= cfg->block && cfg->small_threads % cfg->block == 0 &&
                                    cfg->small_threads < cfg->bfs_threads),
               "the small-graph lane needs small_threads: a multiple of block below bfs_threads");
@@ -848,6 +963,60 @@ extern "C" int gcq_server_start(gcq_server* s) {
   }
   GCQ_HIP(hipMalloc(&s->subq, qbytes));
   GCQ_HIP(hipMalloc(&s->desc, sizeof(sv::Req) * (S + J)));
+  // Multi-source groups: ceil(S / 64) per graph, so any S queries can be in
+  // flight on one graph; seen / nxt[0] / nxt[1] words per group and vertex.
+  std::vector<bk::Group> hg;
+  uint64_t gwords = 0;
+  if (c.multi_source) {
+    const int32_t per_graph = (int32_t)((S + 63) / 64);
+    s->graph_groups.assign(s->graphs.size(), {});
+    uint64_t lane_items[sv::kLanes] = {0, 0};
+    for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
+      const uint64_t nv = (uint64_t)s->graphs[gi]->host.nv;
+      for (int32_t k = 0; k < per_graph; ++k) {
+        s->graph_groups[gi].push_back((int32_t)hg.size());
+        bk::Group G{};
+        G.off = (uint64_t)s->graphs[gi]->d_off;
+        G.idx = (uint64_t)s->graphs[gi]->d_idx;
+        G.seen = gwords;  // word offsets until gbits exists
+        G.nxt[0] = gwords + nv;
+        G.nxt[1] = gwords + 2 * nv;
+        gwords += 3 * nv;
+        hg.push_back(G);
+        lane_items[s->glane[gi]] += nv;  // one (group, vertex) item per level at most
+      }
+    }
+    GCQ_REQUIRE(hg.size() <= (1ull << (31 - s->vbits)), "%zu groups do not fit beside %u-bit vertices", hg.size(),
+                s->vbits);
+    for (int l = 0; l < sv::kLanes; ++l)
+      GCQ_REQUIRE(lane_items[l] <= s->lane_cap[l], "lane %d's frontier ring (%llu) is below its groups' %llu vertices",
+                  l, (unsigned long long)s->lane_cap[l], (unsigned long long)lane_items[l]);
+    const uint64_t ng = hg.size();
+    GCQ_HIP(hipMalloc(&s->groups, sizeof(bk::Group) * ng));
+    GCQ_HIP(hipMalloc(&s->gctl, sizeof(sv::GroupCtl) * ng));
+    GCQ_HIP(hipMalloc(&s->gdepth, 8ull * 64 * ng));
+    GCQ_HIP(hipMalloc(&s->gstart, 4ull * 64 * ng));
+    GCQ_HIP(hipMalloc(&s->gslot, 4ull * 64 * ng));
+    GCQ_HIP(hipMalloc(&s->gbits, 8ull * gwords));
+    std::vector<sv::GroupCtl> hc(ng);
+    for (uint64_t k = 0; k < ng; ++k) {
+      hg[k].seen = (uint64_t)(s->gbits + hg[k].seen);
+      hg[k].nxt[0] = (uint64_t)(s->gbits + hg[k].nxt[0]);
+      hg[k].nxt[1] = (uint64_t)(s->gbits + hg[k].nxt[1]);
+      hg[k].depth = (uint64_t)(s->gdepth + 64 * k);
+      hg[k].start = (uint64_t)(s->gstart + 64 * k);
+      hg[k].fresh = (uint64_t)&s->gctl[k].fresh[0];
+      hc[k] = sv::GroupCtl{};
+    }
+    for (size_t gi = 0; gi < s->graphs.size(); ++gi)
+      for (int32_t k : s->graph_groups[gi]) {
+        hc[k].lane = (uint32_t)s->glane[gi];
+        hc[k].nv = (uint32_t)s->graphs[gi]->host.nv;
+      }
+    GCQ_HIP(hipMemcpy(s->groups, hg.data(), sizeof(bk::Group) * ng, hipMemcpyHostToDevice));
+    GCQ_HIP(hipMemcpy(s->gctl, hc.data(), sizeof(sv::GroupCtl) * ng, hipMemcpyHostToDevice));
+    s->gbusy.assign(ng, 0ull);
+  }
   GCQ_HIP(hipMalloc(&s->arena_base, c.budget_bytes));
   s->arena.init(c.budget_bytes);
   if (c.keep_bytes) {
@@ -882,6 +1051,12 @@ extern "C" int gcq_server_start(gcq_server* s) {
     hipLaunchKernelGGL(sv::k_init_rings, dim3(gcqf::grid_of(2ull * qn[q])), dim3(256), 0, s->stream,
                        s->subq + s->subq_off[q], (uint64_t)0, qn[q], 1u);
   zero(s->desc, sizeof(sv::Req) * (S + J));
+  if (c.multi_source) {
+    zero(s->gbits, 8ull * gwords);
+    zero(s->gdepth, 8ull * 64 * hg.size());
+    zero(s->gstart, 4ull * 64 * hg.size());
+    zero(s->gslot, 4ull * 64 * hg.size());
+  }
   GCQ_HIP(hipGetLastError());
   GCQ_HIP(hipStreamSynchronize(s->stream));
 
@@ -908,6 +1083,14 @@ extern "C" int gcq_server_start(gcq_server* s) {
   d.bfs_blocks = c.bfs_threads / c.block;
   d.lane_blocks[0] = lane_threads[0] / c.block;
   d.lane_blocks[1] = lane_threads[1] / c.block;
+  d.ms = c.multi_source ? 1u : 0u;
+  d.ngroups = (uint32_t)hg.size();
+  d.md = bk::MsDev{s->groups, &s->ctl->err, (uint32_t)hg.size(), s->vbits, c.pop_width, c.flush_at};
+  d.groups = s->groups;
+  d.gctl = s->gctl;
+  d.gdepth = s->gdepth;
+  d.gstart = s->gstart;
+  d.gslot = s->gslot;
   d.max_requests = c.max_requests;
   d.rt_workers = s->scene ? 1u : 0u;
   // The gate wave leads the first block after the BFS sub-grid; a server

@@ -70,7 +70,7 @@ class ServerConfig(C.Structure):
                 ("bfs_slots", C.c_uint32), ("rt_slots", C.c_uint32), ("frontier_log2", C.c_uint32),
                 ("max_depth", C.c_int32), ("pop_width", C.c_uint32), ("flush_at", C.c_uint32),
                 ("budget_bytes", C.c_uint64), ("keep_bytes", C.c_uint64), ("max_requests", C.c_uint32),
-                ("small_log2", C.c_uint32), ("small_threads", C.c_uint32), ("pad", C.c_uint32)]
+                ("small_log2", C.c_uint32), ("small_threads", C.c_uint32), ("multi_source", C.c_uint32)]
 
 
 _P = C.POINTER

@@ -448,7 +448,9 @@ typedef struct gcq_server_config {
   uint32_t small_log2;     /* small-graph lane: queries on graphs of at most 2^small_log2 vertices run in a
                               second BFS sub-grid with its own level barrier (0 = one lane for all) */
   uint32_t small_threads;  /* the second lane's threads, taken out of bfs_threads (multiple of block) */
-  uint32_t pad;
+  uint32_t multi_source;   /* 1: queries on one graph share bit-parallel groups of 64 (ceil(bfs_slots / 64)
+                              groups per graph, 24 B per vertex each, outside budget_bytes); each query's
+                              depths stay bit-exact.  0: one (query, vertex) item per discovery */
 } gcq_server_config;
 void gcq_server_config_default(gcq_server_config* cfg);
 /* graphs and scene are borrowed and must outlive the server; scene may be

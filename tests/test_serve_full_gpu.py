@@ -3,7 +3,8 @@
 request budget and the benchmark's server configuration (a 2^28-slot
 frontier ring, 245,760 BFS + 16,384 RT threads in blocks of 512), with one
 BFS lane and with 8,192 of the BFS threads in a second lane for the graphs
-of at most 2^18 vertices.  About 60
+of at most 2^18 vertices, with and without bit-parallel groups for the
+queries on one graph.  About 60
 requests, 90% BFS queries on sources in each graph's giant component, 10%
 256 x 256 tiles, served both as a Poisson trace at 150 requests/s and as one
 burst.  Every BFS depth array equals the serial BFS oracle's bit for bit;
@@ -18,7 +19,8 @@ FRAME = (1920, 1080)
 SCALES = list(range(14, 23))
 SERVER = dict(bfs_threads=245760, block=512, rt_threads=16384, frontier_log2=28, budget_bytes=64 << 30,
               keep_bytes=4 << 30)
-LANES = {1: {}, 2: dict(small_log2=18, small_threads=8192)}
+LANES = {1: {}, 2: dict(small_log2=18, small_threads=8192), "ms": dict(small_log2=18, small_threads=8192,
+                                                                       multi_source=1)}
 
 
 @pytest.fixture(scope="module")
@@ -36,7 +38,7 @@ def big_world(gcq, orc):
         g.close()
 
 
-@pytest.mark.parametrize("lanes", [1, 2], ids=["one_lane", "two_lanes"])
+@pytest.mark.parametrize("lanes", [1, 2, "ms"], ids=["one_lane", "two_lanes", "two_lanes_ms"])
 @pytest.mark.parametrize("rate", [150.0, 1e9], ids=["poisson150", "burst"])
 def test_cfg5_scale_mix_is_exact(gcq, orc, big_world, rate, lanes):
     from gcq_amd import serve

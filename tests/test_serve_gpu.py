@@ -48,10 +48,12 @@ def _check_results(orc, srv, rows, comps, graphs, scene, csrs, full_frame=None):
 
 # Two lanes: queries on graphs of at most 2^11 vertices (R-MAT scales 9 and
 # 11) run in a second BFS sub-grid of 4,096 threads with its own barrier.
-LANES = {1: {}, 2: dict(small_log2=11, small_threads=4096)}
+# "ms": two lanes, and queries on one graph share bit-parallel groups.
+LANES = {1: {}, 2: dict(small_log2=11, small_threads=4096), "ms": dict(small_log2=11, small_threads=4096,
+                                                                       multi_source=1)}
 
 
-@pytest.mark.parametrize("lanes", [1, 2], ids=["one_lane", "two_lanes"])
+@pytest.mark.parametrize("lanes", [1, 2, "ms"], ids=["one_lane", "two_lanes", "two_lanes_ms"])
 def test_mix_equals_alone(gcq, orc, world, lanes):
     from gcq_amd import serve
     graphs, scene, csrs = world
@@ -66,7 +68,7 @@ def test_mix_equals_alone(gcq, orc, world, lanes):
         _check_results(orc, srv, rows, comps, graphs, scene, csrs, full)
 
 
-@pytest.mark.parametrize("lanes", [1, 2], ids=["one_lane", "two_lanes"])
+@pytest.mark.parametrize("lanes", [1, 2, "ms"], ids=["one_lane", "two_lanes", "two_lanes_ms"])
 @pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["gwfq", "glfq", "wfq", "sfq"])
 def test_burst_with_tight_limits(gcq, orc, world, kind, lanes):
     # Everything arrives at once; 3 BFS slots, 2 tile slots and a budget of
@@ -79,7 +81,7 @@ def test_burst_with_tight_limits(gcq, orc, world, kind, lanes):
     budget = 2 * 4 * max(nvs) + 2 * 128 * 128 * 76 + 4096
     print(f"[serve burst] kind={kind} lanes={lanes}", flush=True)
     with serve.Server(graphs, scene, FRAME, kind=kind, budget_bytes=budget, keep_bytes=1 << 28, bfs_slots=3,
-                      rt_slots=2, bfs_threads=8192 if lanes == 2 else 4096, rt_threads=4096, frontier_log2=20,
+                      rt_slots=2, bfs_threads=4096 if lanes == 1 else 8192, rt_threads=4096, frontier_log2=20,
                       **LANES[lanes]) as srv:
         comps = srv.run_trace(rows)
         _check_results(orc, srv, rows, comps, graphs, scene, csrs)
