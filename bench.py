@@ -287,7 +287,7 @@ def profile_run(a):
         with rt.Scene.preset(rt.RANDOM, 64, 3) as scene:
             for _ in range(a.runs):
                 with serve.Server(graphs, scene, (1920, 1080), bfs_threads=245760, block=512, rt_threads=16384,
-                                  frontier_log2=28, small_log2=18, small_threads=8192) as srv:
+                                  frontier_log2=28, small_log2=18, small_threads=8192, multi_source=1) as srv:
                     res = serve.latency_summary(srv.run_trace(rows))
         out.update(res)
         for gr in graphs:
@@ -445,9 +445,9 @@ def secondary(g, with_cpu):
     # two BFS lanes: graphs of at most 2^18 vertices (scales 14..18) get
     # 8,192 of the BFS threads and their own level barrier; tiles 16,384
     # threads (a sweep of splits on MI355X: this one keeps the one-lane p99
-    # and cuts p50 about 10x)
+    # and cuts p50 about 10x); queries on one graph share bit-parallel groups
     scfg = dict(bfs_threads=245760, block=512, rt_threads=16384, frontier_log2=28, small_log2=18,
-                small_threads=8192)
+                small_threads=8192, multi_source=1)
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         comps = srv.run_trace(rows)
         s_trace = serve.latency_summary(comps)
@@ -457,20 +457,34 @@ def secondary(g, with_cpu):
         by_lane["tiles"] = serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 1])
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         s_burst = serve.latency_summary(srv.run_trace(burst))
-    one = {k: v for k, v in scfg.items() if k not in ("small_log2", "small_threads")}
+    # the same trace without groups, with two lanes and with one
+    per_q = dict(scfg, multi_source=0)
+    with serve.Server(graphs, scene, frame, **per_q) as srv:
+        s_perq = serve.latency_summary(srv.run_trace(rows))
+    with serve.Server(graphs, scene, frame, **per_q) as srv:
+        s_perq_burst = serve.latency_summary(srv.run_trace(burst))
+    one = {k: v for k, v in per_q.items() if k not in ("small_log2", "small_threads")}
     with serve.Server(graphs, scene, frame, **one) as srv:
         s_one = serve.latency_summary(srv.run_trace(rows))
     pcfg = dict(bfs_threads=65536, rt_threads_per_tile=8192)
     p_trace = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows, **pcfg))
     p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, **pcfg))
+    # a second rate, near the plain path's burst capacity, where its requests queue
+    lam2 = 250.0
+    rows2 = serve.poisson_trace(300, lam2, nvs, frame, rt_fraction=0.1, seed=7, graph_sources=pick)
+    with serve.Server(graphs, scene, frame, **scfg) as srv:
+        s_trace2 = serve.latency_summary(srv.run_trace(rows2))
+    p_trace2 = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows2, **pcfg))
     rnd = lambda d: {k: round(v, 3) if isinstance(v, float) else v for k, v in d.items()}  # noqa: E731
     out["served"] = {
         "workload": f"Poisson {lam:.0f} req/s, 300 requests: 90% BFS on R-MAT scales 14..22, 10% 256x256 tiles "
                     "of the 1920x1080 64-sphere scene (4 bounces), 64 GB budget, arrival seed 5",
         "served": rnd(s_trace), "plain": rnd(p_trace),
         "served_by_class": {k: rnd(v) for k, v in by_lane.items()},
-        "served_one_lane": rnd(s_one),
+        "served_per_query_items": rnd(s_perq), "burst_served_per_query_items": rnd(s_perq_burst),
+        "served_one_lane_per_query_items": rnd(s_one),
         "burst_served": rnd(s_burst), "burst_plain": rnd(p_burst),
+        f"served_at_{lam2:.0f}_per_s": rnd(s_trace2), f"plain_at_{lam2:.0f}_per_s": rnd(p_trace2),
         "capacity_vs_plain": round(s_burst["throughput_rps"] / p_burst["throughput_rps"], 2),
         "config": dict(scfg, plain_bfs_threads=pcfg["bfs_threads"],
                        plain_rt_threads_per_tile=pcfg["rt_threads_per_tile"])}
