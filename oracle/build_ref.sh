@@ -7,7 +7,11 @@
 #   dropin_<harness>_<queue>  the same harness compiled against this
 #                             repository's drop-in headers (include/*.hpp);
 #   dropinrlx_rt_<queue>      the ray tracing harness on the drop-in headers
-#                             without the dequeue acquire (GCQ_DROPIN_RELAXED).
+#                             without the dequeue acquire (GCQ_DROPIN_RELAXED);
+#   refnc_rt_glfq             the reference's ray tracing harness and G-LFQ
+#                             queue compiled with -ffp-contract=off, the
+#                             arithmetic of this repository's renderer, so its
+#                             PPM can be compared byte for byte.
 #                             oracle/_ref/dsrc/ mirrors the reference layout
 #                             with symbolic links only: the drop-in headers at
 #                             its top, links to the harness sources in their
@@ -60,9 +64,10 @@ build_one() {  # flavour harness queue-index
   rel=$(rel_of "$h")
   bin=$OUT/${flav}_${h}_${TAGS[$qi]}
   log=$OUT/build_${flav}_${h}_${TAGS[$qi]}.log
-  if [ "$flav" = ref ]; then src=$REF/$rel; else src=$DS/$rel; fi
+  if [ "$flav" = ref ] || [ "$flav" = refnc ]; then src=$REF/$rel; else src=$DS/$rel; fi
   local extra=""
   [ "$flav" = dropinrlx ] && extra="-DGCQ_DROPIN_RELAXED"
+  [ "$flav" = refnc ] && extra="-ffp-contract=off"
   for d in "$REF/$rel" "$ROOT"/include/*.hpp "$ROOT"/gcq_amd/csrc/gcq/*.hpp; do
     t=$(stat -L -c %Y "$d"); [ "$t" -gt "$newest" ] && newest=$t
   done
@@ -80,6 +85,7 @@ for qi in 0 1 2 3; do items+=("dropinrlx rt $qi"); done  # drop-in headers witho
 for spec in "tb 0" "tb 1" "tb 2" "tb 3" "hist 0" "hist 1" "hist 2" "hist 3" "bfs 0" "bfs 1" "rt 0" "rt 1"; do
   items+=("ref $spec")
 done
+items+=("refnc rt 1")
 status=0
 pids=()
 for it in "${items[@]}"; do
@@ -89,5 +95,5 @@ for it in "${items[@]}"; do
   while [ "$(jobs -rp | wc -l)" -ge "${GCQ_BUILD_JOBS:-8}" ]; do sleep 0.2; done
 done
 for p in "${pids[@]}"; do wait "$p" || status=1; done
-echo "build_ref: $(ls "$OUT" | grep -cE '^(ref|dropin|dropinrlx)_[a-z]+_[a-z]+$') binaries in oracle/_ref"
+echo "build_ref: $(ls "$OUT" | grep -cE '^(ref|refnc|dropin|dropinrlx)_[a-z]+_[a-z]+$') binaries in oracle/_ref"
 exit $status
