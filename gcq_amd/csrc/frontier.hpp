@@ -221,11 +221,11 @@ struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt
     const uint64_t T = gcq::wave_faa(Hp(q), (uint64_t)1);
     if (T >= end) return -1;
     if (T >= q->g.cells) return 0;
-    Cell* c = &cells(q)[T];
-    uint64_t x = gcq::ld_rlx(&c->val);
+    uint64_t* const c = &vals(q)[T];
+    uint64_t x = gcq::ld_rlx(c);
     for (uint32_t w = 0; x == kBottom && w < gcq::kSlotWaitSpins; ++w) {
       __builtin_amdgcn_s_sleep(1);
-      x = gcq::ld_rlx(&c->val);
+      x = gcq::ld_rlx(c);
     }
     if (x == kBottom || x == kTop || x == kRetry) return 0;
     *v = (uint32_t)(x - 1);

@@ -53,12 +53,22 @@ constexpr uint64_t kRetry = 0xFFFFFFFFFFFFFFFEull;
 constexpr uint32_t kRefNull = 0u, kRefTop = 1u;    // request references
 constexpr uint32_t kHdrBytes = 2048;
 
-struct Cell {
-  uint64_t val;
-  uint32_t enq;  // request reference
-  uint32_t deq;
+// A cell is 16 B in two arrays indexed alike: its value (u64) in the first,
+// its enqueue and dequeue request references (u32 each) in the second.  An
+// enqueue's fast path touches only the value and a dequeue's the value and
+// the dequeue reference, so the 64 consecutive cells of a wave's tickets
+// span 4 lines of each array (a device-scope atomic is one request per line
+// per wave instruction) instead of 8 lines of interleaved 16-B cells.
+struct Refs {
+  uint32_t enq, deq;
 };
-static_assert(sizeof(Cell) == 16, "cell");
+static_assert(sizeof(Refs) == 8, "refs");
+struct Cell {
+  uint64_t* val;
+  uint32_t* enq;
+  uint32_t* deq;
+  __device__ explicit operator bool() const { return val != nullptr; }
+};
 
 // Per-thread handle (reference wf_handle, :161-178).  Requests are read by
 // helpers; each handle is one 64-B half line.
@@ -118,9 +128,16 @@ struct Queue {
 static_assert(sizeof(Queue) == kHdrBytes, "header");
 
 __host__ __device__ constexpr uint64_t queue_bytes(uint64_t cells) { return kHdrBytes + 16ull * cells; }
-__device__ __forceinline__ Cell* cells(Queue* q) {
+// The values: g.cells u64 words after the header (or at ext_cells), then
+// the reference pairs, g.cells more u64 words.
+__device__ __forceinline__ uint64_t* vals(Queue* q) {
   const uint64_t ext = q->g.ext_cells;
-  return ext ? (Cell*)ext : (Cell*)((char*)q + kHdrBytes);
+  return ext ? (uint64_t*)ext : (uint64_t*)((char*)q + kHdrBytes);
+}
+__device__ __forceinline__ Refs* refs(Queue* q) { return (Refs*)(vals(q) + q->g.cells); }
+__device__ __forceinline__ Cell cell_at(Queue* q, uint64_t k) {
+  Refs* r = refs(q) + k;
+  return Cell{vals(q) + k, &r->enq, &r->deq};
 }
 __device__ __forceinline__ uint64_t* Tp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_t); }
 __device__ __forceinline__ uint64_t* Hp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_h); }
@@ -252,10 +269,11 @@ struct Ops {
   // could orphan the item its enqueuer writes into that cell later, so it
   // waits kPatientRounds times longer than an enqueuer (which may give its
   // ticket up: the cell's dequeuer then marks the cell).
-  __device__ __forceinline__ Cell* cell(uint64_t i, bool patient = false) {
-    if (!ring) return i < ncells ? &cells(q)[i] : nullptr;
-    if ((i >> seg_log2) >= avail && !wait_window(i >> seg_log2, patient ? kPatientRounds : 1u)) return nullptr;
-    return &cells(q)[i & (ncells - 1)];
+  __device__ __forceinline__ Cell cell(uint64_t i, bool patient = false) {
+    if (!ring) return i < ncells ? cell_at(q, i) : Cell{nullptr, nullptr, nullptr};
+    if ((i >> seg_log2) >= avail && !wait_window(i >> seg_log2, patient ? kPatientRounds : 1u))
+      return Cell{nullptr, nullptr, nullptr};
+    return cell_at(q, i & (ncells - 1));
   }
 
   // ---------------------------------------------------------------- ring mode
@@ -441,7 +459,8 @@ struct Ops {
     const uint64_t m = active_mask();
     const uint32_t L = first_lane(m), n = (uint32_t)__builtin_popcountll(m), r = rank_in(m);
     ClearJob* j = job(q);
-    uint64_t* wd = (uint64_t*)cells(q);
+    uint64_t* const vw = vals(q);
+    uint64_t* const rw = (uint64_t*)refs(q);
     for (uint64_t t = 0; t < chunks; ++t) {
       uint64_t c0 = 0, c1 = 0, e = 0;
       if (lane_id() == L) {
@@ -464,8 +483,8 @@ struct Ops {
       c1 = bcast64(c1, L);
       if (c1 == c0) return false;
       for (uint64_t c = c0 + r; c < c1; c += n) {
-        st_rlx(&wd[2 * (c & (ncells - 1))], 0ull);
-        st_rlx(&wd[2 * (c & (ncells - 1)) + 1], 0ull);
+        st_rlx(&vw[c & (ncells - 1)], 0ull);
+        st_rlx(&rw[c & (ncells - 1)], 0ull);
       }
       vm_wait();  // this chunk is clear before it is counted
       uint32_t fin = 0;
@@ -490,9 +509,9 @@ struct Ops {
   __device__ __forceinline__ void advance_end(uint64_t* e, uint64_t cid) {
     if (ld_rlx(e) < cid) max_rlx(e, cid);  // usually already past: a load, no RMW
   }
-  __device__ __forceinline__ void enq_commit(Cell* c, uint64_t v, uint64_t cid) {  // :259-267
+  __device__ __forceinline__ void enq_commit(Cell c, uint64_t v, uint64_t cid) {  // :259-267
     advance_end(Tp(q), cid + 1);
-    xchg_rlx(&c->val, v);
+    xchg_rlx(c.val, v);
   }
   __device__ __forceinline__ bool try_claim(uint64_t* st, uint64_t id, uint64_t cid) {
     return cas_once(st, pack_state(1, id), pack_state(0, cid));
@@ -509,13 +528,13 @@ struct Ops {
     pr->after_ticket();
     if (ring && lane_id() == first_lane(active_mask())) wave_lds()->last_t[wave_in_block()] = i;
     maybe_reclaim(i);
-    Cell* c = cell(i);
+    Cell c = cell(i);
     *cid = i;
     if (!c) {
       *cid = kWithdrawn;  // no cell: the pool is exhausted
       return false;
     }
-    return cas_once(&c->val, kBottom, v);
+    return cas_once(c.val, kBottom, v);
   }
 
   __device__ bool enq_slow(uint64_t v, uint64_t cell_id) {  // :283-318
@@ -526,15 +545,15 @@ struct Ops {
     for (;;) {
       pr->step();
       const uint64_t i = faa_rlx(Tp(q), (uint64_t)1);
-      Cell* c = cell(i);
+      Cell c = cell(i);
       if (!c) {
         // Pool exhausted: withdraw the request unless a helper has already
         // claimed a cell for it (then that cell receives the value below).
         if (cas_once(&h->enq_state, pack_state(1, cell_id), pack_state(0, kWithdrawn))) return false;
         break;
       }
-      if (cas_once(&c->enq, kRefNull, ref_of(self))) {
-        const uint64_t cur = ld_rlx(&c->val);
+      if (cas_once(c.enq, kRefNull, ref_of(self))) {
+        const uint64_t cur = ld_rlx(c.val);
         if (cur == kBottom || cur == kTop) {
           (void)try_claim(&h->enq_state, cell_id, i);
           break;
@@ -543,7 +562,7 @@ struct Ops {
       if (!st_pending(ld_rlx(&h->enq_state))) break;
     }
     const uint64_t fid = st_id(ld_rlx(&h->enq_state));
-    Cell* d = cell(fid);
+    Cell d = cell(fid);
     if (!d) return false;  // pool exhausted
     // The enqueuer writes its value into the cell claimed for its request
     // whatever the cell holds (⊥ or ⊤), as YMC's enq_slow does.  The
@@ -558,10 +577,10 @@ struct Ops {
   }
 
   // Resolve cell i for a dequeuer: its value, EMPTY or RETRY (:320-413).
-  __device__ uint64_t help_enq(Cell* c, uint64_t i) {
+  __device__ uint64_t help_enq(Cell c, uint64_t i) {
     uint64_t old = kBottom;
-    if (!cas_rlx(&c->val, old, kTop) && old != kTop) return old;
-    uint32_t ce = ld_rlx(&c->enq);
+    if (!cas_rlx(c.val, old, kTop) && old != kTop) return old;
+    uint32_t ce = ld_rlx(c.enq);
     if (ce == kRefNull) {
       uint32_t p = h->enq_peer;
       Handle* ph = &hs[p];
@@ -575,7 +594,7 @@ struct Ops {
       }
       if (st_pending(s) && st_id(s) > 0 && st_id(s) <= i) {
         uint32_t prev = kRefNull;
-        cas_rlx(&c->enq, prev, ref_of(p));
+        cas_rlx(c.enq, prev, ref_of(p));
         if (prev != kRefNull && prev != ref_of(p)) {
           ce = prev;
           h->enq_id_last = st_id(s);  // keep tracking this peer
@@ -590,7 +609,7 @@ struct Ops {
       }
       if (ce == kRefNull) {
         uint32_t prev = kRefNull;
-        ce = cas_rlx(&c->enq, prev, kRefTop) ? kRefTop : prev;
+        ce = cas_rlx(c.enq, prev, kRefTop) ? kRefTop : prev;
       }
     }
     if (ce == kRefTop) return ld_rlx(Tp(q)) <= i ? kEmpty : kRetry;
@@ -600,16 +619,16 @@ struct Ops {
     const uint64_t v = ld_rlx(&rh->enq_val);
     if (!st_pending(s)) {
       if (st_id(s) == i) {
-        if (ld_rlx(&c->val) == kTop) enq_commit(c, v, i);
-        return ld_rlx(&c->val);
+        if (ld_rlx(c.val) == kTop) enq_commit(c, v, i);
+        return ld_rlx(c.val);
       }
       return ld_rlx(Tp(q)) <= i ? kEmpty : kRetry;
     }
     if (st_id(s) > i) return ld_rlx(Tp(q)) <= i ? kEmpty : kRetry;
     if (try_claim(&rh->enq_state, st_id(s), i) || ld_rlx(&rh->enq_state) == pack_state(0, i)) {
-      if (ld_rlx(&c->val) == kTop) enq_commit(c, v, i);
+      if (ld_rlx(c.val) == kTop) enq_commit(c, v, i);
     }
-    return ld_rlx(&c->val);
+    return ld_rlx(c.val);
   }
 
   // Head ticket bounded by the cells enqueuers have taken.  The wave leader
@@ -649,18 +668,18 @@ struct Ops {
   __device__ uint64_t deq_fast(uint64_t i) {  // :416-434, ticket from claim_h
     pr->step();
     pr->after_ticket();
-    Cell* c = cell(i, true);
+    Cell c = cell(i, true);
     if (!c) return kEmpty;
     // The ticket lies below T: its enqueuer is in flight.  Give it a moment
     // before help_enq marks the cell TOP and sends the enqueuer round again.
-    for (uint32_t w = 0; w < kSlotWaitSpins && ld_rlx(&c->val) == kBottom; ++w) {
+    for (uint32_t w = 0; w < kSlotWaitSpins && ld_rlx(c.val) == kBottom; ++w) {
       __builtin_amdgcn_s_sleep(1);
       pr->step();
     }
     const uint64_t v = help_enq(c, i);
     if (v == kEmpty) return kEmpty;
     if (v != kTop && v != kRetry) {
-      if (cas_once(&c->deq, kRefNull, kRefTop)) return v;
+      if (cas_once(c.deq, kRefNull, kRefTop)) return v;
     }
     return kRetry;
   }
@@ -676,11 +695,11 @@ struct Ops {
       s = ld_rlx(&helpee->deq_state);
       if (!st_pending(s) || ld_rlx(&helpee->deq_id) != id) return;
       while (!cand && st_pending(s) && st_id(s) == prior) {
-        Cell* c = cell(++i);
+        Cell c = cell(++i);
         if (!c) break;
         advance_end(Hp(q), i + 1);
         const uint64_t v = help_enq(c, i);
-        if (v == kEmpty || (v != kTop && v != kRetry && ld_rlx(&c->deq) == kRefNull)) {
+        if (v == kEmpty || (v != kTop && v != kRetry && ld_rlx(c.deq) == kRefNull)) {
           cand = i;
           break;
         }
@@ -694,11 +713,11 @@ struct Ops {
       s = ld_rlx(&helpee->deq_state);
       if (!st_pending(s) || ld_rlx(&helpee->deq_id) != id) return;
       const uint64_t aidx = st_id(s);
-      Cell* c = cell(aidx);
+      Cell c = cell(aidx);
       if (!c) return;
-      const uint32_t cur = ld_rlx(&c->deq);
+      const uint32_t cur = ld_rlx(c.deq);
       const uint32_t mine = ref_of(helpee_tid);
-      if (ld_rlx(&c->val) == kTop || cas_once(&c->deq, kRefNull, mine) || cur == mine) {
+      if (ld_rlx(c.val) == kTop || cas_once(c.deq, kRefNull, mine) || cur == mine) {
         cas_once(&helpee->deq_state, s, pack_state(0, aidx));
         return;
       }
@@ -714,10 +733,10 @@ struct Ops {
     st_rlx(&h->deq_state, pack_state(1, cid));
     help_deq(h, self);
     const uint64_t idx = st_id(ld_rlx(&h->deq_state));
-    Cell* c = cell(idx);
+    Cell c = cell(idx);
     advance_end(Hp(q), idx + 1);
     if (!c) return kEmpty;
-    const uint64_t v = ld_rlx(&c->val);
+    const uint64_t v = ld_rlx(c.val);
     return v == kTop ? kEmpty : v;
   }
 
@@ -854,8 +873,12 @@ __device__ __forceinline__ uint64_t high_water(Queue* q) {
 // are cleared with device-scope stores so waves on other XCDs see them.
 __device__ inline void recycle_cells(Queue* q, uint64_t gid, uint64_t stride) {
   const uint64_t hw = min(high_water(q), q->g.cells);
-  uint64_t* w = (uint64_t*)cells(q);
-  for (uint64_t i = gid; i < 2 * hw; i += stride) st_rlx(&w[i], 0ull);
+  uint64_t* const vw = vals(q);
+  uint64_t* const rw = (uint64_t*)refs(q);
+  for (uint64_t i = gid; i < hw; i += stride) {
+    st_rlx(&vw[i], 0ull);
+    st_rlx(&rw[i], 0ull);
+  }
 }
 __device__ inline void reset_handle(Queue* q, Handle* hs, uint32_t tid) {
   const uint32_t n = q->g.nthreads;
