@@ -157,6 +157,9 @@ def main():
         "accesses_per_s_M": round((mix["atomics_per_s"] + mix["loads_per_s"]) / 1e6, 2),
         "faa_per_s_M_pollers_on_other_line": round(other["atomics_per_s"] / 1e6, 2),
         "faa_per_s_M_alone": round(p1["atomics_per_s"] / 1e6, 2)}
+    # latency of one dependent device-scope access (the bound of
+    # level-synchronous work such as the lattice BFS)
+    peaks["round_trip_ns"] = {m: round(g.round_trip(m), 1) for m in g.ROUND_TRIP_MODES}
 
     # Consumers poll without sleeping between empty attempts (poll_backoff 0):
     # measured fastest for this split (backoff 1 and 2 are 6% and 9% slower
@@ -350,12 +353,18 @@ def secondary(g, with_cpu):
         best = min(per_q, key=per_q.get)
         med = per_q[best]
         alg_bytes = 28 * res["reached"] + 8 * res["edges_scanned"]
+        rt_ns = g.round_trip("load")
         o = {"workload": f"lattice 4900x4900, keep 0.75, diag 0.01, seed 7, src 0, 8192 threads, persistent, "
                          f"{QUEUE_NAMES[g.KINDS[best]]} frontier (fastest of the four)",
              "vertices": nv, "edges": ne, "levels": res["levels"], "reached": res["reached"],
              "edges_scanned": res["edges_scanned"], "queue": best, "kernel_ms_median": round(med, 3),
              "mteps": round(res["edges_scanned"] / med / 1e3, 1),
              "per_queue_ms": {k: round(v, 3) for k, v in per_q.items()},
+             # each level is a chain of dependent device-scope accesses: its
+             # time in units of one measured round trip
+             "latency_bound": {"per_level_us": round(med * 1e3 / res["levels"], 3),
+                               "round_trip_ns": round(rt_ns, 1),
+                               "round_trips_per_level": round(med * 1e6 / res["levels"] / rt_ns, 2)},
              "roofline": {"bound": "hbm", "achieved": round(alg_bytes / med / 1e6, 2), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(alg_bytes / med / 1e6 / HBM_PEAK_GBS, 5),
                           "bytes_per_launch": alg_bytes}}

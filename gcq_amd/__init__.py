@@ -108,6 +108,7 @@ def load():
         "gcq_faa_peak": (C.c_int, [_P(FaaConfig), _P(FaaResult)]),
         "gcq_fence_cost": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P(C.c_double)]),
         "gcq_hbm_copy": (C.c_int, [C.c_uint64, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
+        "gcq_round_trip": (C.c_int, [C.c_uint32, C.c_uint32, _P(C.c_double)]),
         "gcq_dropin_selfcheck": (C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, _P(C.c_uint32)]),
         "gcq_tb_run": (C.c_int, [C.c_void_p, _P(TbConfig), _P(TbResult)]),
         "gcq_queue_drain": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint64)]),
@@ -254,6 +255,19 @@ def faa_peak(threads=65536, block=256, counters=1, stride_bytes=128, aggregate=T
 def fence_cost(mode, threads=65536, block=256, iters=512):
     ns = C.c_double(0)
     _check(load().gcq_fence_cost(threads, block, mode, iters, C.byref(ns)), "gcq_fence_cost")
+    return ns.value
+
+
+ROUND_TRIP_MODES = ("load", "atomic", "cached_load")
+
+
+def round_trip(mode, steps=20000):
+    """Nanoseconds per dependent device-scope access (mode: index or name in
+    ROUND_TRIP_MODES) along a random cycle of 65,536 lines."""
+    if isinstance(mode, str):
+        mode = ROUND_TRIP_MODES.index(mode)
+    ns = C.c_double(0)
+    _check(load().gcq_round_trip(mode, steps, C.byref(ns)), "gcq_round_trip")
     return ns.value
 
 

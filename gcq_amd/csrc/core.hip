@@ -366,6 +366,59 @@ extern "C" int gcq_hbm_copy(uint64_t bytes, uint32_t iters, double* gbps, double
   return GCQ_OK;
 }
 
+namespace {
+// One lane follows a random cycle through `n` words spaced one 128-B line
+// apart, each step's address taken from the previous step's result: the
+// latency of one dependent device-scope access.  mode 0: device-scope load,
+// 1: device-scope fetch-add of 0 (an atomic served on the memory side),
+// 2: load through the caches.
+__global__ void k_chase(uint64_t* buf, uint32_t steps, uint32_t mode, uint64_t* sink) {
+  if (threadIdx.x != 0) return;
+  uint64_t i = 0;
+  for (uint32_t k = 0; k < steps; ++k) {
+    uint64_t* p = buf + 16 * i;
+    if (mode == 0) i = gcq::ld_rlx(p);
+    else if (mode == 1) i = gcq::faa_rlx(p, (uint64_t)0);
+    else i = gcq::ld_cached(p);
+  }
+  *sink = i;
+}
+}  // namespace
+
+extern "C" int gcq_round_trip(uint32_t mode, uint32_t steps, double* ns_per_step) {
+  GCQ_REQUIRE(ns_per_step && mode <= 2 && steps >= 16, "bad argument");
+  constexpr uint32_t n = 1u << 16;  // 8 MiB of lines: each step a line not touched for ~n steps
+  std::vector<uint64_t> h(16ull * n, 0);
+  std::vector<uint32_t> order(n);
+  for (uint32_t k = 0; k < n; ++k) order[k] = k;
+  uint64_t x = 0x9E3779B97F4A7C15ull;  // fixed shuffle: one cycle through every line
+  for (uint32_t k = n - 1; k > 0; --k) {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    std::swap(order[k], order[x % (k + 1)]);
+  }
+  for (uint32_t k = 0; k < n; ++k) h[16ull * order[k]] = order[(k + 1) % n];
+  uint64_t* d = nullptr;
+  GCQ_HIP(hipMalloc(&d, 8ull * h.size() + 64));
+  GCQ_HIP(hipMemcpy(d, h.data(), 8ull * h.size(), hipMemcpyHostToDevice));
+  hipEvent_t a, b;
+  GCQ_HIP(hipEventCreate(&a));
+  GCQ_HIP(hipEventCreate(&b));
+  uint64_t* sink = d + h.size();
+  hipLaunchKernelGGL(k_chase, dim3(1), dim3(64), 0, 0, d, 256u, mode, sink);  // warm
+  GCQ_HIP(hipEventRecord(a, 0));
+  hipLaunchKernelGGL(k_chase, dim3(1), dim3(64), 0, 0, d, steps, mode, sink);
+  GCQ_HIP(hipEventRecord(b, 0));
+  GCQ_HIP(hipEventSynchronize(b));
+  GCQ_HIP(hipGetLastError());
+  *ns_per_step = gcqh::ms_between(a, b) * 1e6 / steps;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(d);
+  return GCQ_OK;
+}
+
 extern "C" int gcq_fence_cost(uint32_t threads, uint32_t block, uint32_t mode, uint32_t iters, double* ns) {
   GCQ_REQUIRE(ns && threads && block && iters && mode <= 3, "bad argument");
   uint64_t* d = nullptr;

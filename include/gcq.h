@@ -103,6 +103,13 @@ int gcq_faa_peak(const gcq_faa_config* cfg, gcq_faa_result* res);
  * 1 store + release fence, 2 store + acquire fence, 3 store + seq_cst fence. */
 int gcq_fence_cost(uint32_t threads, uint32_t block, uint32_t mode, uint32_t iters, double* ns_per_iter);
 
+/* Latency of one dependent device-scope access: one lane follows a random
+ * cycle through 65,536 lines, each address from the previous result.  mode
+ * 0: device-scope load, 1: device-scope atomic (fetch-add of 0), 2: load
+ * through the caches.  The latency bound of level-synchronous work (a BFS
+ * level is a chain of such accesses).  No reference counterpart. */
+int gcq_round_trip(uint32_t mode, uint32_t steps, double* ns_per_step);
+
 /* Streaming copy of `bytes` (read + write counted) averaged over iters
  * launches: the measured HBM roofline, and a kernel of known traffic for
  * calibrating HBM byte counters.  No reference counterpart. */
