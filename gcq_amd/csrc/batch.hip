@@ -84,7 +84,7 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
 
 // Multi-source groups (bfs_batch.hpp): the same level loop over group items.
 template <class F>
-__global__ void k_batch_ms(Frontiers<F> fq, MsDev d, Ctl* ctl) {
+__global__ void k_batch_ms(Frontiers<F> fq, MsDev d, Ctl* ctl, HubPool* hub) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
@@ -96,15 +96,27 @@ __global__ void k_batch_ms(Frontiers<F> fq, MsDev d, Ctl* ctl) {
     if constexpr (F::kQueues == 2)
       if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
     uint64_t end = 0;
+    HubPool* const hp = &hub[L & 1];
     const uint32_t dw = wave_sum32(ms_expand<F>(fq.in(L), fq.out(L), d, L, tid, st,
-                                                F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end));
+                                                F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end,
+                                                hp));
     prev_end[L & 1] = end;
     if (lane_id() == 0 && dw) gcq::faa_rlx(&ctl->next[L % 3], dw);
+    auto level_end = [&]() {
+      if (F::kQueues == 3) fq.idle(L).rewind();
+      hub_reset(&hub[(L + 1) & 1]);
+    };
     if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
-          if (F::kQueues == 3) fq.idle(L).rewind();
+          if (gcq::ld_rlx(&hp->word) == 0ull) level_end();  // else the level ends after its hub scan
         }))
       break;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (gcq::ld_rlx(&hp->word) != 0ull) {
+      const uint32_t dh = wave_sum32(ms_hubs(fq.out(L), d, hp, L, tid, (uint32_t)(nthreads / 64), st));
+      if (lane_id() == 0 && dh) gcq::faa_rlx(&ctl->next[L % 3], dh);
+      if (!grid_barrier(&ctl->bar, gridDim.x, level_end)) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
     const uint32_t n = gcq::ld_rlx(&ctl->next[L % 3]);
     if (blockIdx.x == 0 && threadIdx.x == 0) gcq::st_rlx(&ctl->next[(L + 2) % 3], 0u);
     if (n == 0) break;
@@ -186,6 +198,7 @@ struct gcq_batch {
   int32_t* done = nullptr;
   uint64_t* stats = nullptr;
   bk::Ctl* ctl = nullptr;
+  bk::HubPool* hubs = nullptr;  // device: the two level-parity pools of the multi-source kernel
   // pinned read-back
   int32_t* h_done = nullptr;
   uint64_t* h_stats = nullptr;
@@ -271,6 +284,7 @@ int ensure_buffers(gcq_batch* b, uint32_t nq, uint64_t total_v, uint64_t visited
   }
   if (!b->ctl) {
     GCQ_HIP(hipMalloc(&b->ctl, sizeof(bk::Ctl)));
+    GCQ_HIP(hipMalloc(&b->hubs, 2 * sizeof(bk::HubPool)));
     GCQ_HIP(hipHostMalloc((void**)&b->h_ctl, sizeof(bk::Ctl)));
   }
   return GCQ_OK;
@@ -304,7 +318,10 @@ int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq, const bk::Dev& d, 
     const bk::MsDev md{b->ms_groups, &b->ctl->err, b->ms_ngroups, b->vbits, cfg->pop_width, cfg->flush_at};
     hipLaunchKernelGGL(bk::k_seed_ms<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], md, (const uint32_t*)b->ms_qinfo,
                        nq);
-    hipLaunchKernelGGL(bk::k_batch_ms<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, md, b->ctl);
+    for (int p = 0; p < 2; ++p)
+      hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, s, (uint64_t*)(b->hubs + p), bk::kHubHeaderWords);
+    hipLaunchKernelGGL(bk::k_batch_ms<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, md, b->ctl,
+                       b->hubs);
   } else {
     hipLaunchKernelGGL(bk::k_seed<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], d);
     hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d, b->ctl);
@@ -707,6 +724,7 @@ extern "C" int gcq_batch_destroy(gcq_batch* b) {
   if (b->depth) (void)hipFree(b->depth);
   if (b->visited) (void)hipFree(b->visited);
   if (b->ctl) (void)hipFree(b->ctl);
+  if (b->hubs) (void)hipFree(b->hubs);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;

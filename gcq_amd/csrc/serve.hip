@@ -164,6 +164,7 @@ struct Dev {
   uint64_t* gdepth;
   int32_t* gstart;
   uint32_t* gslot;
+  bk::HubPool* hub;  // per BFS lane, the two level-parity pools of long adjacencies (multi-source form)
 };
 
 __device__ void publish(const Dev& d, uint64_t id, int32_t type, int32_t levels, int32_t slot, uint64_t rays) {
@@ -401,17 +402,32 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
     // the level after it, and every block drops its CU's and XCD's cached
     // lines after each barrier.
     uint64_t end = 0;
+    bk::HubPool* const hp = d.ms ? &d.hub[2 * lane + (L & 1)] : nullptr;
     if (d.ms)
       (void)bk::ms_expand<F, true>(fq.in(L), fq.out(L), d.md, L, tid, stage,
-                                   F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
+                                   F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end, hp);
     else
       (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage,
                                  F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
     prev_end[L & 1] = end;
-    if (!grid_barrier(&lc->bar, nblocks, [&]() {
-          if (F::kQueues == 3) fq.idle(L).rewind();
-          admit_bfs(d, lane, L);
-        })) {
+    auto level_end = [&]() {
+      if (F::kQueues == 3) fq.idle(L).rewind();
+      if (hp) bk::hub_reset(&d.hub[2 * lane + ((L + 1) & 1)]);
+      admit_bfs(d, lane, L);
+    };
+    // A level that posted long adjacencies ends after the lane's scan of
+    // them, at a second barrier.
+    bool ok = grid_barrier(&lc->bar, nblocks, [&]() {
+      if (!hp || gcq::ld_rlx(&hp->word) == 0ull) level_end();
+    });
+    if (ok) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (hp && gcq::ld_rlx(&hp->word) != 0ull) {
+        (void)bk::ms_hubs(fq.out(L), d.md, hp, L, tid, (uint32_t)(nthreads / 64), stage);
+        ok = grid_barrier(&lc->bar, nblocks, level_end);
+      }
+    }
+    if (!ok) {
       if (threadIdx.x == 0) gcq::st_rlx(&d.c->exit, 1u);  // the gate and the RT waves leave too
       break;
     }
@@ -638,6 +654,7 @@ struct gcq_server {
   int32_t* gstart = nullptr;
   uint32_t* gslot = nullptr;
   uint64_t* gbits = nullptr;         // seen, nxt[0], nxt[1] of every group
+  bk::HubPool* hubs = nullptr;       // per BFS lane, two level-parity pools of long adjacencies
   std::vector<std::vector<int32_t>> graph_groups;  // group indices of each graph
   std::vector<uint64_t> gbusy;       // members in flight, per group (host view)
   char* arena_base = nullptr;  // in-flight request state, budget_bytes
@@ -672,7 +689,7 @@ void release_all(gcq_server* s) {
   for (void* p : {(void*)s->slots, (void*)s->alive, (void*)s->slot_active, (void*)s->dgraphs, (void*)s->jobs,
                   (void*)s->rings, (void*)s->ctl, (void*)s->subq, (void*)s->desc, (void*)s->arena_base,
                   (void*)s->keep_base, (void*)s->groups, (void*)s->gctl, (void*)s->gdepth, (void*)s->gstart,
-                  (void*)s->gslot, (void*)s->gbits})
+                  (void*)s->gslot, (void*)s->gbits, (void*)s->hubs})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)s->sub, (void*)s->comp, (void*)s->stop})
     if (p) (void)hipHostFree(p);
@@ -998,6 +1015,7 @@ extern "C" int gcq_server_start(gcq_server* s) {
     GCQ_HIP(hipMalloc(&s->gstart, 4ull * 64 * ng));
     GCQ_HIP(hipMalloc(&s->gslot, 4ull * 64 * ng));
     GCQ_HIP(hipMalloc(&s->gbits, 8ull * gwords));
+    GCQ_HIP(hipMalloc(&s->hubs, 2ull * sv::kLanes * sizeof(bk::HubPool)));
     std::vector<sv::GroupCtl> hc(ng);
     for (uint64_t k = 0; k < ng; ++k) {
       hg[k].seen = (uint64_t)(s->gbits + hg[k].seen);
@@ -1056,6 +1074,7 @@ extern "C" int gcq_server_start(gcq_server* s) {
     zero(s->gdepth, 8ull * 64 * hg.size());
     zero(s->gstart, 4ull * 64 * hg.size());
     zero(s->gslot, 4ull * 64 * hg.size());
+    zero(s->hubs, 2ull * sv::kLanes * sizeof(bk::HubPool));
   }
   GCQ_HIP(hipGetLastError());
   GCQ_HIP(hipStreamSynchronize(s->stream));
@@ -1091,6 +1110,7 @@ extern "C" int gcq_server_start(gcq_server* s) {
   d.gdepth = s->gdepth;
   d.gstart = s->gstart;
   d.gslot = s->gslot;
+  d.hub = s->hubs;
   d.max_requests = c.max_requests;
   d.rt_workers = s->scene ? 1u : 0u;
   // The gate wave leads the first block after the BFS sub-grid; a server
