@@ -92,7 +92,7 @@ template <bool kCounted, class F, bool kBitmap = false>
 __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restrict__ offs,
                              const int32_t* __restrict__ idx, int32_t* level, int32_t L, uint32_t tid, uint32_t* stage,
                              LevelAcc& acc, uint32_t* err, uint64_t h0 = kNoHead, uint64_t grid_lanes = 0,
-                             uint64_t* end_out = nullptr, uint32_t* vis = nullptr) {
+                             uint64_t* end_out = nullptr, uint32_t* vis = nullptr, HubPool* hp = nullptr) {
   const uint32_t lane = lane_id();
   uint32_t sc = 0;  // staged vertices (wave-uniform)
   uint64_t end = 0;
@@ -135,6 +135,7 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
       deg = offs[u + 1] - beg;
     }
     acc.edges += (uint64_t)deg;
+    if (hp && deg > kHubDeg && hub_post(hp, u, beg, deg, 0ull)) deg = 0;  // the grid scans it
     int32_t incl = deg;  // inclusive prefix of degrees across the wave
     for (int o = 1; o < 64; o <<= 1) {
       const int32_t t = __shfl_up(incl, o, 64);
@@ -193,6 +194,46 @@ __device__ void expand_level(const F& qin, const F& qout, const int32_t* __restr
   if (sc) flush(qout, tid, stage, sc, sc, err);
 }
 
+// The grid's scan of a level's posted vertices (between the level's two
+// barriers), as expand_level scans an edge: a probe of the target's depth
+// (or visited bit) and a claim.
+template <class F, bool kBitmap>
+__device__ void hub_level(const F& qout, const int32_t* __restrict__ idx, int32_t* level, int32_t L, uint32_t tid,
+                          uint32_t nwaves, uint32_t* stage, LevelAcc& acc, uint32_t* err, uint32_t* vis,
+                          HubPool* hp) {
+  const uint32_t lane = lane_id();
+  uint32_t sc = 0;
+  hub_scan(hp, tid, nwaves, [&](const HubEntry* e, int32_t first, int32_t stop) {
+    const int32_t* adj = idx + gcq::ld_cached(&e->beg);
+    int32_t e0 = first + (int32_t)lane;
+    uint32_t w_next = e0 < stop ? (uint32_t)adj[e0] : 0u;
+    for (; e0 - (int32_t)lane < stop; e0 += 64) {
+      const uint32_t w = w_next;
+      if (e0 + 64 < stop) w_next = (uint32_t)adj[e0 + 64];
+      bool disc = false;
+      if (e0 < stop) {
+        if constexpr (kBitmap) {
+          uint32_t* const vw = vis + (w >> 5);
+          const uint32_t vb = 1u << (w & 31u);
+          if (!(gcq::ld_cached(vw) & vb)) {
+            disc = !(gcq::or_rlx(vw, vb) & vb);
+            if (disc) level[w] = L + 1;
+          }
+        } else if (gcq::ld_cached(&level[w]) == -1) {
+          int32_t expect = -1;
+          disc = gcq::cas_rlx(&level[w], expect, L + 1);
+        }
+      }
+      const uint64_t dm = __ballot(disc);
+      if (disc) stage[sc + gcq::rank_in(dm)] = w;
+      sc += (uint32_t)__builtin_popcountll(dm);
+      acc.disc += disc ? 1u : 0u;
+      if (sc >= 64) flush(qout, tid, stage, sc, 64, err);
+    }
+  });
+  if (sc) flush(qout, tid, stage, sc, sc, err);
+}
+
 template <class F>
 __device__ void finish_counts(LevelAcc& acc, uint64_t disc_total, Ctl* c) {
   const uint64_t e = gcq::wave_sum(acc.edges);
@@ -205,7 +246,7 @@ __device__ void finish_counts(LevelAcc& acc, uint64_t disc_total, Ctl* c) {
 
 template <class F, bool kBitmap = false>
 __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, const int32_t* __restrict__ idx,
-                             int32_t* level, Ctl* c, uint32_t* vis) {
+                             int32_t* level, Ctl* c, uint32_t* vis, HubPool* hub) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
@@ -214,6 +255,7 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
   uint64_t disc_total = 0;
   uint64_t prev_end[2] = {kNoHead, kNoHead};  // per frontier: its end when last drained
   uint32_t cum = 0;                            // discoveries over all levels so far
+  uint32_t seq = 0;                            // level barriers passed
   int32_t L = 0;
   for (;; ++L) {
     if (F::kQueues == 3) fq.idle(L).recycle(tid, nthreads, tid);
@@ -225,20 +267,41 @@ __global__ void k_persistent(Frontiers<F> fq, const int32_t* __restrict__ offs, 
       if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
     acc.disc = 0;
     uint64_t end = 0;
+    HubPool* const hp = hub ? &hub[L & 1] : nullptr;
     expand_level<F::kCounted, F, kBitmap>(fq.in(L), fq.out(L), offs, idx, level, L, tid, st, acc, &c->err,
-                                          F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads, &end, vis);
+                                          F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads, &end, vis, hp);
     prev_end[L & 1] = end;
     disc_total += acc.disc;
+    // A level that posted long adjacencies ends after the grid's scan of
+    // them, at a second barrier; either way the level's last barrier
+    // clears the other parity's pool.
+    auto level_end = [&]() -> uint32_t {
+      uint32_t f = 0;
+      if constexpr (F::kQueues == 3) {
+        fq.idle(L).rewind();
+        f = kBarrierStores;
+      }
+      if (hub) {
+        hub_reset(&hub[(L + 1) & 1]);
+        f = kBarrierStores;
+      }
+      return f;
+    };
     uint32_t n = 0;
-    if (!level_barrier(&c->lbar, gridDim.x, L, wave_sum32(acc.disc), &n, &cum, [&]() {
-          if constexpr (F::kQueues == 3) {
-            fq.idle(L).rewind();
-            return true;
-          } else {
-            return false;
-          }
-        }))
+    bool hubs = false;
+    if (!level_barrier(&c->lbar, gridDim.x, seq++, wave_sum32(acc.disc), &n, &cum, [&]() -> uint32_t {
+          return hp && gcq::ld_rlx(&hp->word) != 0ull ? kBarrierHubs : level_end();
+        }, &hubs))
       break;
+    if (hubs) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the posted entries, not stale lines
+      acc.disc = 0;
+      hub_level<F, kBitmap>(fq.out(L), idx, level, L, tid, (uint32_t)(nthreads / 64), st, acc, &c->err, vis, hp);
+      disc_total += acc.disc;
+      uint32_t n2 = 0;
+      if (!level_barrier(&c->lbar, gridDim.x, seq++, wave_sum32(acc.disc), &n2, &cum, level_end)) break;
+      n += n2;
+    }
     if (n == 0) break;
   }
   finish_counts<F>(acc, disc_total, c);
@@ -434,6 +497,7 @@ int ctx_free(gcq_graph* g) {
   if (c.level) (void)hipFree(c.level);
   if (c.vis) (void)hipFree(c.vis);
   if (c.ctl) (void)hipFree(c.ctl);
+  if (c.hub) (void)hipFree(c.hub);
   if (c.pinned) (void)hipHostFree(c.pinned);
   c = BfsCtx();
   return GCQ_OK;
@@ -448,6 +512,7 @@ int ctx_ensure(gcq_graph* g, int kind, uint32_t threads) {
     GCQ_HIP(hipMalloc(&c.level, 4ull * nv));
     GCQ_HIP(hipMalloc(&c.vis, vis_bytes(nv)));
     GCQ_HIP(hipMalloc(&c.ctl, sizeof(bfsk::Ctl)));
+    GCQ_HIP(hipMalloc(&c.hub, 2 * sizeof(gcqf::HubPool)));
     GCQ_HIP(hipHostMalloc((void**)&c.pinned, 64));
   }
   return GCQ_OK;
@@ -462,6 +527,9 @@ int reset_run(gcq_graph* g, int32_t src, bool bitmap) {
                        (uint64_t*)c.vis, vis_bytes(nv) / 8);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, g->stream, (uint64_t*)c.ctl,
                      (uint64_t)(sizeof(bfsk::Ctl) / 8));
+  for (int p = 0; p < 2; ++p)
+    hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, g->stream, (uint64_t*)((gcqf::HubPool*)c.hub + p),
+                       gcqf::kHubHeaderWords);
   c.fs.reset(g->stream);
   c.fs.dispatch([&](const auto& fq) {
     using F = std::decay_t<decltype(fq.q[0])>;
@@ -497,6 +565,15 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
     }
   } else if (cfg->mode == GCQ_BFS_PERSISTENT || cfg->mode == GCQ_BFS_BITMAP) {
     const bool bitmap = cfg->mode == GCQ_BFS_BITMAP;
+    // Long adjacencies are shared out by the grid only where the graph has
+    // one: without a pool, a level costs no look at it.
+    if (c.max_deg < 0) {
+      int32_t m = 0;
+      const auto& o = g->host.offsets;
+      for (size_t v = 0; v + 1 < o.size(); ++v) m = std::max(m, o[v + 1] - o[v]);
+      c.max_deg = m;
+    }
+    gcqf::HubPool* const hub = c.max_deg > gcqf::kHubDeg ? (gcqf::HubPool*)c.hub : nullptr;
     int per_cu = 0, dev = 0, cus = 0;
     GCQ_HIP(hipGetDevice(&dev));
     GCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -509,10 +586,10 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
       GCQ_HIP(hipEventRecord(e0, g->stream));
       if (bitmap)
         hipLaunchKernelGGL((bfsk::k_persistent<F, true>), dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off,
-                           g->d_idx, c.level, ctl, c.vis);
+                           g->d_idx, c.level, ctl, c.vis, hub);
       else
         hipLaunchKernelGGL((bfsk::k_persistent<F, false>), dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off,
-                           g->d_idx, c.level, ctl, (uint32_t*)nullptr);
+                           g->d_idx, c.level, ctl, (uint32_t*)nullptr, hub);
       GCQ_HIP(hipEventRecord(e1, g->stream));
     }
   } else {
@@ -769,7 +846,7 @@ extern "C" int gcq_bfs_run_noalloc(const int32_t* d_offsets, const int32_t* d_in
   hipLaunchKernelGGL(gcqf::kf_reset_lfq, dim3(gcqf::grid_of(2ull * cap)), dim3(256), 0, s, ring0, ring1, cap);
   hipLaunchKernelGGL(bfsk::k_seed<gcqf::FLfq>, dim3(1), dim3(64), 0, s, fq.q[0], d_depth, src, ctl, (uint32_t*)nullptr);
   hipLaunchKernelGGL(bfsk::k_persistent<gcqf::FLfq>, dim3(grid), dim3(block), 0, s, fq, d_offsets, d_indices, d_depth,
-                     ctl, (uint32_t*)nullptr);
+                     ctl, (uint32_t*)nullptr, (gcqf::HubPool*)nullptr);
   GCQ_HIP(hipGetLastError());
   bfsk::Ctl h;
   GCQ_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, s));

@@ -226,112 +226,28 @@ struct MsDev {
   uint32_t ngroups, vbits, pop_width, flush_at;
 };
 
-// ---------------------------------------------------------------- hub vertices
-// A vertex whose adjacency is long (R-MAT hubs reach 162,818 edges at
-// scale 22) would hold its level open while one wave scans it 64 edges a
-// round.  A wave that settles such a vertex posts it instead, and after the
-// level barrier the whole grid scans the posted edges in chunks, then meets
-// at a second barrier; a level that posts nothing keeps its one barrier.
-// (Enterprise's split of the frontier by degree, Liu & Huang, SC'15, with
-// the grid taking the largest class.)  Pools alternate by level parity: a
-// level's pool is read by every block after the level's first barrier and
-// cleared at the end of the next level, when no block can still read it.
-constexpr int32_t kHubDeg = 4096;     // more edges than this: posted
-constexpr uint32_t kHubChunk = 1024;  // edges per claimed chunk
-constexpr uint32_t kMaxHubs = 2048;   // entries per level; a vertex past them is scanned by its wave
-constexpr uint32_t kHubCursors = 8;   // chunk claims striped over 8 lines
-
-struct HubEntry {
-  uint32_t item, base;  // (group, vertex); its first chunk
-  int32_t beg, deg;     // its edge range
-  uint64_t fresh;       // the members sent along its edges
-};
-struct HubPool {
-  uint64_t word, p0[15];  // [63:40] entries, [39:0] chunks, posted this level
-  uint32_t cursor[kHubCursors][32];
-  HubEntry e[kMaxHubs];
-};
-constexpr uint64_t kHubHeaderWords = (128 + 128 * kHubCursors) / 8;  // what a reset clears
-static_assert(sizeof(HubPool) == 8 * kHubHeaderWords + sizeof(HubEntry) * kMaxHubs, "hub pool");
-
-// Post one vertex; false when the level's table is full.
-__device__ __forceinline__ bool hub_post(HubPool* hp, uint32_t item, int32_t beg, int32_t deg, uint64_t fresh) {
-  const uint64_t nch = ((uint64_t)deg + kHubChunk - 1) / kHubChunk;
-  uint64_t w = gcq::ld_rlx(&hp->word);
-  for (;;) {
-    if ((w >> 40) >= kMaxHubs) return false;
-    if (gcq::cas_rlx(&hp->word, w, w + (1ull << 40) + nch)) break;
-  }
-  HubEntry* e = &hp->e[w >> 40];
-  gcq::st_rlx(&e->item, item);
-  gcq::st_rlx(&e->base, (uint32_t)(w & ((1ull << 40) - 1)));
-  gcq::st_rlx(&e->beg, beg);
-  gcq::st_rlx(&e->deg, deg);
-  gcq::st_rlx(&e->fresh, fresh);
-  return true;
-}
-
-// Level end: clear the other parity's pool (last used a level ago, next
-// used a level from now).  Run by one thread, before a barrier's release.
-__device__ __forceinline__ void hub_reset(HubPool* hp) {
-  gcq::st_rlx(&hp->word, 0ull);
-  for (uint32_t c = 0; c < kHubCursors; ++c) gcq::st_rlx(&hp->cursor[c][0], 0u);
-}
-
-// The grid's scan of a level's posted vertices, between the level's two
-// barriers (all threads of every block call it; the pool is final).  Wave
-// w < P (P = min(chunks, waves)) first takes chunk w, then chunks
-// P + s + 8k by fetch-add on cursor s, moving to the next cursor when one
-// runs out.  Returns the calling lane's discoveries.
+// The grid's scan of a level's posted vertices, multi-source form: each
+// edge sends the posted vertex's fresh members to the target's gathering
+// word.  Returns the calling lane's discoveries.
 template <class F>
 __device__ uint32_t ms_hubs(const F& qout, const MsDev& d, HubPool* hp, int32_t L, uint32_t tid, uint32_t nwaves,
                             uint32_t* stage) {
-  __shared__ uint32_t s_base[kMaxHubs];
-  const uint64_t word = gcq::ld_rlx(&hp->word);
-  const uint32_t n = (uint32_t)(word >> 40);
-  const uint64_t total = word & ((1ull << 40) - 1);
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_base[i] = gcq::ld_rlx(&hp->e[i].base);
-  __syncthreads();
   const uint32_t lane = lane_id();
-  const uint32_t wv = tid / 64;
-  const uint64_t P = total < nwaves ? total : nwaves;
   uint32_t sc = 0, disc_n = 0;
-  if (wv >= P) return 0;
-  uint64_t c = wv;
-  uint32_t s = wv % kHubCursors, tried = 0;
-  for (;;) {
-    while (c >= total && tried < kHubCursors) {
-      uint32_t k = 0;
-      if (lane == 0) k = gcq::faa_rlx(&hp->cursor[s][0], 1u);
-      c = P + s + (uint64_t)kHubCursors * __shfl(k, 0, 64);
-      if (c >= total) {
-        s = (s + 1) % kHubCursors;
-        ++tried;
-      }
-    }
-    if (c >= total) break;
-    uint32_t lo = 0, hi = n;  // last entry whose first chunk is at or below c
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (s_base[mid] <= c) lo = mid;
-      else hi = mid;
-    }
-    const HubEntry* e = &hp->e[lo];
+  hub_scan(hp, tid, nwaves, [&](const HubEntry* e, int32_t first, int32_t stop) {
     const uint32_t item = gcq::ld_cached(&e->item);
-    const int32_t beg = gcq::ld_cached(&e->beg), deg = gcq::ld_cached(&e->deg);
+    const int32_t beg = gcq::ld_cached(&e->beg);
     const uint64_t fresh = gcq::ld_cached(&e->fresh);
     const uint32_t g = item >> d.vbits;
     const Group* G = &d.groups[g];
-    const int32_t* idx = (const int32_t*)gcq::ld_cached(&G->idx);
+    const int32_t* idx = (const int32_t*)gcq::ld_cached(&G->idx) + beg;
     const uint64_t* seen = (const uint64_t*)gcq::ld_cached(&G->seen);
     uint64_t* out = (uint64_t*)gcq::ld_cached(&G->nxt[L & 1]);
-    const int32_t first = (int32_t)(c - s_base[lo]) * (int32_t)kHubChunk;
-    const int32_t stop = first + (int32_t)kHubChunk < deg ? first + (int32_t)kHubChunk : deg;
     int32_t e0 = first + (int32_t)lane;
-    uint32_t w_next = e0 < stop ? (uint32_t)idx[beg + e0] : 0u;
+    uint32_t w_next = e0 < stop ? (uint32_t)idx[e0] : 0u;
     for (; e0 - (int32_t)lane < stop; e0 += 64) {
       const uint32_t w = w_next;
-      if (e0 + 64 < stop) w_next = (uint32_t)idx[beg + e0 + 64];
+      if (e0 + 64 < stop) w_next = (uint32_t)idx[e0 + 64];
       bool disc = false;
       if (e0 < stop) {
         const uint64_t send = fresh & ~gcq::ld_cached(seen + w);
@@ -343,8 +259,7 @@ __device__ uint32_t ms_hubs(const F& qout, const MsDev& d, HubPool* hp, int32_t 
       disc_n += disc ? 1u : 0u;
       while (sc >= d.flush_at) flush(qout, tid, stage, sc, sc < 64u ? sc : 64u, d.err);
     }
-    c = total;  // the next chunk by claim
-  }
+  });
   if (sc) flush(qout, tid, stage, sc, sc, d.err);
   return disc_n;
 }

@@ -317,6 +317,103 @@ __device__ bool grid_barrier(Barrier* b, uint32_t nblocks, Last last) {
   return gcq::ld_rlx(&b->abort) == 0u;
 }
 
+// ---------------------------------------------------------------- hub vertices
+// A vertex whose adjacency is long (R-MAT hubs reach 162,818 edges at
+// scale 22) would hold its level open while one wave scans it 64 edges a
+// round.  A wave that settles such a vertex posts it instead, and after the
+// level barrier the whole grid scans the posted edges in chunks, then meets
+// at a second barrier; a level that posts nothing keeps its one barrier.
+// (Enterprise's split of the frontier by degree, Liu & Huang, SC'15, with
+// the grid taking the largest class.)  Pools alternate by level parity: a
+// level's pool is read by every block after the level's first barrier and
+// cleared at the end of the next level, when no block can still read it.
+constexpr int32_t kHubDeg = 4096;     // more edges than this: posted
+constexpr uint32_t kHubChunk = 1024;  // edges per claimed chunk
+constexpr uint32_t kMaxHubs = 2048;   // entries per level; a vertex past them is scanned by its wave
+constexpr uint32_t kHubCursors = 8;   // chunk claims striped over 8 lines
+
+struct HubEntry {
+  uint32_t item, base;  // (group, vertex); its first chunk
+  int32_t beg, deg;     // its edge range
+  uint64_t fresh;       // the members sent along its edges
+};
+struct HubPool {
+  uint64_t word, p0[15];  // [63:40] entries, [39:0] chunks, posted this level
+  uint32_t cursor[kHubCursors][32];
+  HubEntry e[kMaxHubs];
+};
+constexpr uint64_t kHubHeaderWords = (128 + 128 * kHubCursors) / 8;  // what a reset clears
+static_assert(sizeof(HubPool) == 8 * kHubHeaderWords + sizeof(HubEntry) * kMaxHubs, "hub pool");
+
+// Post one vertex; false when the level's table is full.  `fresh` is the
+// multi-source form's member mask (unused by the single-source form).
+__device__ __forceinline__ bool hub_post(HubPool* hp, uint32_t item, int32_t beg, int32_t deg, uint64_t fresh) {
+  const uint64_t nch = ((uint64_t)deg + kHubChunk - 1) / kHubChunk;
+  uint64_t w = gcq::ld_rlx(&hp->word);
+  for (;;) {
+    if ((w >> 40) >= kMaxHubs) return false;
+    if (gcq::cas_rlx(&hp->word, w, w + (1ull << 40) + nch)) break;
+  }
+  HubEntry* e = &hp->e[w >> 40];
+  gcq::st_rlx(&e->item, item);
+  gcq::st_rlx(&e->base, (uint32_t)(w & ((1ull << 40) - 1)));
+  gcq::st_rlx(&e->beg, beg);
+  gcq::st_rlx(&e->deg, deg);
+  gcq::st_rlx(&e->fresh, fresh);
+  return true;
+}
+
+// Level end: clear the other parity's pool (last used a level ago, next
+// used a level from now).  Run by one thread, before a barrier's release.
+__device__ __forceinline__ void hub_reset(HubPool* hp) {
+  gcq::st_rlx(&hp->word, 0ull);
+  for (uint32_t c = 0; c < kHubCursors; ++c) gcq::st_rlx(&hp->cursor[c][0], 0u);
+}
+
+// The grid's scan of a level's posted vertices, between the level's two
+// barriers (all threads of every block call it; the pool is final).  Wave
+// w < P (P = min(chunks, waves)) first takes chunk w, then chunks
+// P + s + 8k by fetch-add on cursor s, moving to the next cursor when one
+// runs out.  scan(entry, first, stop) runs wave-wide over the entry's edges
+// [first, stop) (offsets into its adjacency).
+template <class Scan>
+__device__ void hub_scan(HubPool* hp, uint32_t tid, uint32_t nwaves, Scan&& scan) {
+  __shared__ uint32_t s_base[kMaxHubs];
+  const uint64_t word = gcq::ld_rlx(&hp->word);
+  const uint32_t n = (uint32_t)(word >> 40);
+  const uint64_t total = word & ((1ull << 40) - 1);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_base[i] = gcq::ld_rlx(&hp->e[i].base);
+  __syncthreads();
+  const uint32_t wv = tid / 64;
+  const uint64_t P = total < nwaves ? total : nwaves;
+  if (wv >= P) return;
+  uint64_t c = wv;
+  uint32_t s = wv % kHubCursors, tried = 0;
+  for (;;) {
+    while (c >= total && tried < kHubCursors) {
+      uint32_t k = 0;
+      if (lane_id() == 0) k = gcq::faa_rlx(&hp->cursor[s][0], 1u);
+      c = P + s + (uint64_t)kHubCursors * __shfl(k, 0, 64);
+      if (c >= total) {
+        s = (s + 1) % kHubCursors;
+        ++tried;
+      }
+    }
+    if (c >= total) break;
+    uint32_t lo = 0, hi = n;  // last entry whose first chunk is at or below c
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s_base[mid] <= c) lo = mid;
+      else hi = mid;
+    }
+    const HubEntry* e = &hp->e[lo];
+    const int32_t deg = gcq::ld_cached(&e->deg);
+    const int32_t first = (int32_t)(c - s_base[lo]) * (int32_t)kHubChunk;
+    scan(e, first, first + (int32_t)kHubChunk < deg ? first + (int32_t)kHubChunk : deg);
+    c = total;  // the next chunk by claim
+  }
+}
+
 // Level barrier of the persistent BFS: arrival and the level's discovery
 // count travel in one 64-bit FAA per block ([63:32] blocks arrived, [31:0]
 // vertices discovered), and the release word carries the next level's size
@@ -329,18 +426,21 @@ struct LevelBarrier {
   uint32_t abort, p2[31];
 };
 
-// All threads of the block call it; wave_disc is the calling wave's
-// discovery count (wave-uniform).  *next gets the next level's size and
-// *cum (per thread, the same in every thread) the running total of the
-// levels' sizes.  The arrival word only grows: [63:32] counts arrivals over
-// all levels and [31:0] the discoveries over all levels (below 2^32: each
-// vertex is discovered once), so the last block needs no reset store.  The
-// last block to arrive runs last() first; when last() returns true it made
-// stores that must land before the release.  Returns false once any block
-// has aborted (spin limit).
+// All threads of the block call it, with seq = the number of level
+// barriers the kernel has passed; wave_disc is the calling wave's discovery
+// count (wave-uniform).  *next gets the discoveries since the last barrier
+// and *cum (per thread, the same in every thread) the running total.  The
+// arrival word only grows: [63:32] counts arrivals over all barriers and
+// [31:0] the discoveries over all levels (below 2^31: each vertex is
+// discovered once), so the last block needs no reset store.  The last block
+// to arrive runs last() first: kBarrierStores in its answer means it made
+// stores that must land before the release, kBarrierHubs that the level
+// posted long adjacencies (bit 31 of the release word carries it to *hubs).
+// Returns false once any block has aborted (spin limit).
+constexpr uint32_t kBarrierStores = 1, kBarrierHubs = 2;
 template <class Last>
-__device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint32_t wave_disc, uint32_t* next,
-                              uint32_t* cum, Last last) {
+__device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, uint32_t seq, uint32_t wave_disc, uint32_t* next,
+                              uint32_t* cum, Last last, bool* hubs = nullptr) {
   __shared__ uint32_t s_wd[16];
   __shared__ uint32_t s_next, s_ok;
   gcq::vm_wait();  // this wave's pushes have completed
@@ -351,14 +451,16 @@ __device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint
     for (uint32_t k = 0; k < (blockDim.x + 63) / 64; ++k) sum += s_wd[k];
     const uint64_t old = gcq::faa_rlx(&b->arrive, (1ull << 32) | sum);
     uint32_t ok = 1, n = 0;
-    if ((uint32_t)(old >> 32) == (uint32_t)(L + 1) * nblocks - 1u) {
+    if ((uint32_t)(old >> 32) == (seq + 1u) * nblocks - 1u) {
       n = (uint32_t)old + sum - *cum;
-      if (last()) gcq::vm_wait();  // last()'s stores complete before the release
-      gcq::st_rlx(&b->release, ((uint64_t)(uint32_t)(L + 1) << 32) | n);
+      const uint32_t f = last();
+      if (f & kBarrierStores) gcq::vm_wait();  // last()'s stores complete before the release
+      if (f & kBarrierHubs) n |= 1u << 31;
+      gcq::st_rlx(&b->release, ((uint64_t)(seq + 1u) << 32) | n);
     } else {
       for (uint64_t spins = 0;; ++spins) {
         const uint64_t r = gcq::ld_rlx(&b->release);
-        if ((uint32_t)(r >> 32) == (uint32_t)(L + 1)) {
+        if ((uint32_t)(r >> 32) == seq + 1u) {
           n = (uint32_t)r;
           break;
         }
@@ -374,8 +476,9 @@ __device__ bool level_barrier(LevelBarrier* b, uint32_t nblocks, int32_t L, uint
     s_ok = ok;
   }
   __syncthreads();
-  *next = s_next;
-  *cum += s_next;
+  *next = s_next & 0x7FFFFFFFu;
+  *cum += *next;
+  if (hubs) *hubs = (s_next >> 31) != 0u;
   return s_ok != 0;
 }
 

@@ -9,6 +9,8 @@ struct BfsCtx {
   int32_t* level = nullptr;
   uint32_t* vis = nullptr;  // visited bitmap (GCQ_BFS_BITMAP)
   void* ctl = nullptr;
+  void* hub = nullptr;      // two gcqf::HubPool, by level parity (persistent modes)
+  int32_t max_deg = -1;     // the graph's largest degree, -1 until computed
   uint32_t* pinned = nullptr;  // host-visible counters for the host-loop mode
 };
 

@@ -1,7 +1,8 @@
-"""GPU parity of long adjacencies in the multi-source BFS (batched and
-served): a vertex with more than 4,096 edges is posted by the wave that
-settles it and scanned by the whole grid in 1,024-edge chunks between two
-barriers of its level.  The graph below makes every part of that path run:
+"""GPU parity of long adjacencies in the persistent BFS (one traversal, in
+both claim modes) and in the multi-source BFS (batched and served): a
+vertex with more than 4,096 edges is posted by the wave that settles it and
+scanned by the whole grid in 1,024-edge chunks between two barriers of its
+level.  The graph below makes every part of that path run:
 
 - a centre joined to 2,100 hubs, so level 1 posts more vertices than the
   level's table holds (2,048): the rest are scanned by their own waves;
@@ -65,6 +66,20 @@ def hub_world(gcq, orc):
     refs = {s: orc.bfs_serial(off, idx, s) for s in sources}
     yield g, (off, idx), sources, refs
     g.close()
+
+
+@pytest.mark.parametrize("mode", [1, 3], ids=["persistent", "bitmap"])
+@pytest.mark.parametrize("threads", [4096, 65536])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["gwfq", "glfq", "wfq", "sfq"])
+def test_single_hubs(gcq, hub_world, kind, threads, mode):
+    g, (off, idx), sources, refs = hub_world
+    print(f"[hubs single] kind={kind} threads={threads} mode={mode}", flush=True)
+    for s in sources[:6]:
+        ref, reached, es = refs[s]
+        d, r = g.bfs(s, kind=kind, threads=threads, block=256, mode=mode)
+        bad = np.flatnonzero(d != ref)
+        assert len(bad) == 0, f"source {s}: {len(bad)} depths differ, first {bad[:5]}"
+        assert r["reached"] == reached and r["edges_scanned"] == es and r["levels"] == int(ref.max()) + 1
 
 
 @pytest.mark.parametrize("threads", [4096, 65536])
