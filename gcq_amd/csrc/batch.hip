@@ -39,7 +39,7 @@ struct Ctl {
 };
 
 template <class F>
-__global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
+__global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl, HubPool* hub) {
   __shared__ uint32_t stage[16][128];
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
@@ -55,17 +55,29 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl) {
     if constexpr (F::kQueues == 2)
       if (L > 0 && blockIdx.x == 0 && threadIdx.x == 0) fq.in(L - 1).drained(prev_end[(L - 1) & 1]);
     uint64_t end = 0;
+    HubPool* const hp = &hub[L & 1];
     const uint32_t dw = wave_sum32(expand<F, false>(fq.in(L), fq.out(L), d, L, tid, st,
-                                                    F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end));
+                                                    F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end,
+                                                    hp));
     prev_end[L & 1] = end;
     if (lane_id() == 0 && dw) gcq::faa_rlx(&ctl->next[L % 3], dw);
+    auto level_end = [&]() {
+      if (F::kQueues == 3) fq.idle(L).rewind();
+      hub_reset(&hub[(L + 1) & 1]);
+    };
     if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
-          if (F::kQueues == 3) fq.idle(L).rewind();
+          if (gcq::ld_rlx(&hp->word) == 0ull) level_end();  // else the level ends after its hub scan
         }))
       break;
     // Drop this CU's and XCD's cached lines so the cached visited probes of
     // the next level see at least every bit set up to this barrier.
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (gcq::ld_rlx(&hp->word) != 0ull) {
+      const uint32_t dh = wave_sum32(hubs<F, false>(fq.out(L), d, hp, L, tid, (uint32_t)(nthreads / 64), st));
+      if (lane_id() == 0 && dh) gcq::faa_rlx(&ctl->next[L % 3], dh);
+      if (!grid_barrier(&ctl->bar, gridDim.x, level_end)) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
     const uint32_t n = gcq::ld_rlx(&ctl->next[L % 3]);
     if (blockIdx.x == 0) {
       if (threadIdx.x == 0) gcq::st_rlx(&ctl->next[(L + 2) % 3], 0u);
@@ -324,7 +336,10 @@ int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq, const bk::Dev& d, 
                        b->hubs);
   } else {
     hipLaunchKernelGGL(bk::k_seed<F>, dim3(seed_blocks), dim3(64), 0, s, fq.q[0], d);
-    hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d, b->ctl);
+    for (int p = 0; p < 2; ++p)
+      hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(1), dim3(256), 0, s, (uint64_t*)(b->hubs + p), bk::kHubHeaderWords);
+    hipLaunchKernelGGL(bk::k_batch<F>, dim3(cfg->threads / cfg->block), dim3(cfg->block), 0, s, fq, d, b->ctl,
+                       b->hubs);
   }
   const uint32_t bx = std::max<uint32_t>(1, std::min<uint32_t>(64, (max_nv + 255) / 256));
   hipLaunchKernelGGL(bk::k_stats, dim3(bx, nq), dim3(256), 0, s, d, b->stats);

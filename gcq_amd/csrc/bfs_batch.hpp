@@ -58,7 +58,7 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int lane) {
 // end_out receives the level's end.
 template <class F, bool kLive = true>
 __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L, uint32_t tid, uint32_t* stage,
-                           uint64_t h0, uint64_t grid_waves, uint64_t* end_out) {
+                           uint64_t h0, uint64_t grid_waves, uint64_t* end_out, HubPool* hp) {
   auto rd = [](const auto* p) { return kLive ? gcq::ld_rlx(p) : gcq::ld_cached(p); };
   const uint32_t lane = lane_id();
   const uint32_t vmask = (1u << d.vbits) - 1u;
@@ -98,6 +98,7 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
       dn = L - (int32_t)rd(&s->start) + 1;
       beg = off[v];  // CSR arrays are uploaded before launch and never written
       deg = off[v + 1] - beg;
+      if (hp && deg > kHubDeg && hub_post(hp, item, beg, deg, 0ull)) deg = 0;  // the grid scans it
     }
     int32_t incl = deg;  // inclusive prefix of degrees across the wave
     for (int o = 1; o < 64; o <<= 1) {
@@ -175,6 +176,50 @@ __device__ uint32_t expand(const F& qin, const F& qout, const Dev& d, int32_t L,
     }
     if (one_round) break;  // every ticket of the level was handed out in this round
   }
+  if (sc) flush(qout, tid, stage, sc, sc, d.err);
+  return disc_n;
+}
+
+// The grid's scan of a level's posted vertices, per-query form (between the
+// level's two barriers): each edge is probed and claimed as expand() does.
+// Returns the calling lane's discoveries.
+template <class F, bool kLive = true>
+__device__ uint32_t hubs(const F& qout, const Dev& d, HubPool* hp, int32_t L, uint32_t tid, uint32_t nwaves,
+                         uint32_t* stage) {
+  auto rd = [](const auto* p) { return kLive ? gcq::ld_rlx(p) : gcq::ld_cached(p); };
+  const uint32_t lane = lane_id();
+  uint32_t sc = 0, disc_n = 0;
+  hub_scan(hp, tid, nwaves, [&](const HubEntry* e, int32_t first, int32_t stop) {
+    const uint32_t item = gcq::ld_cached(&e->item);
+    const uint32_t slot = item >> d.vbits;
+    const Slot* s = &d.slots[slot];
+    const int32_t* adj = (const int32_t*)rd(&s->idx) + gcq::ld_cached(&e->beg);
+    int32_t* const dep = (int32_t*)rd(&s->depth);
+    uint32_t* const vis = kLive ? nullptr : (uint32_t*)rd(&s->visited);
+    const int32_t dn = L - (int32_t)rd(&s->start) + 1;
+    int32_t e0 = first + (int32_t)lane;
+    uint32_t w_next = e0 < stop ? (uint32_t)adj[e0] : 0u;
+    for (; e0 - (int32_t)lane < stop; e0 += 64) {
+      const uint32_t w = w_next;
+      if (e0 + 64 < stop) w_next = (uint32_t)adj[e0 + 64];
+      bool disc = false;
+      if (e0 < stop) {
+        uint32_t* vw = vis ? vis + (w >> 5) : nullptr;
+        const uint32_t vb = 1u << (w & 31u);
+        if (vw ? !(gcq::ld_cached(vw) & vb) : rd(dep + w) == -1) {
+          int32_t expect = -1;
+          disc = gcq::cas_rlx(dep + w, expect, dn);
+          if (disc && vw) gcq::or_rlx(vw, vb);
+        }
+      }
+      const uint64_t dm = __ballot(disc);
+      if (dm && lane == (uint32_t)__builtin_ctzll(dm)) gcq::st_rlx(&d.alive[slot * kAliveStride], (uint32_t)(L + 1));
+      if (disc) stage[sc + gcq::rank_in(dm)] = (slot << d.vbits) | w;
+      sc += (uint32_t)__builtin_popcountll(dm);
+      disc_n += disc ? 1u : 0u;
+      while (sc >= d.flush_at) flush(qout, tid, stage, sc, sc < 64u ? sc : 64u, d.err);
+    }
+  });
   if (sc) flush(qout, tid, stage, sc, sc, d.err);
   return disc_n;
 }

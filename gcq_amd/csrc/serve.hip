@@ -402,28 +402,29 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
     // the level after it, and every block drops its CU's and XCD's cached
     // lines after each barrier.
     uint64_t end = 0;
-    bk::HubPool* const hp = d.ms ? &d.hub[2 * lane + (L & 1)] : nullptr;
+    bk::HubPool* const hp = &d.hub[2 * lane + (L & 1)];
     if (d.ms)
       (void)bk::ms_expand<F, true>(fq.in(L), fq.out(L), d.md, L, tid, stage,
                                    F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end, hp);
     else
       (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage,
-                                 F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end);
+                                 F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end, hp);
     prev_end[L & 1] = end;
     auto level_end = [&]() {
       if (F::kQueues == 3) fq.idle(L).rewind();
-      if (hp) bk::hub_reset(&d.hub[2 * lane + ((L + 1) & 1)]);
+      bk::hub_reset(&d.hub[2 * lane + ((L + 1) & 1)]);
       admit_bfs(d, lane, L);
     };
     // A level that posted long adjacencies ends after the lane's scan of
     // them, at a second barrier.
     bool ok = grid_barrier(&lc->bar, nblocks, [&]() {
-      if (!hp || gcq::ld_rlx(&hp->word) == 0ull) level_end();
+      if (gcq::ld_rlx(&hp->word) == 0ull) level_end();
     });
     if (ok) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (hp && gcq::ld_rlx(&hp->word) != 0ull) {
-        (void)bk::ms_hubs(fq.out(L), d.md, hp, L, tid, (uint32_t)(nthreads / 64), stage);
+      if (gcq::ld_rlx(&hp->word) != 0ull) {
+        if (d.ms) (void)bk::ms_hubs(fq.out(L), d.md, hp, L, tid, (uint32_t)(nthreads / 64), stage);
+        else (void)bk::hubs<F, false>(fq.out(L), d.bd, hp, L, tid, (uint32_t)(nthreads / 64), stage);
         ok = grid_barrier(&lc->bar, nblocks, level_end);
       }
     }
@@ -969,6 +970,7 @@ extern "C" int gcq_server_start(gcq_server* s) {
   s->ring_stride = (gcq::lfq::ring_bytes(sv::kMaxTilePixels) + 255) & ~255ull;
   GCQ_HIP(hipMalloc(&s->rings, s->ring_stride * J));
   GCQ_HIP(hipMalloc(&s->ctl, sizeof(sv::Ctl)));
+  GCQ_HIP(hipMalloc(&s->hubs, 2ull * sv::kLanes * sizeof(bk::HubPool)));
   // Submission queues (BFS lane 0, BFS lane 1, RT): a slot is queued at
   // most once, so capacity = slots.
   const uint32_t qn[3] = {gcqf::pow2_at_least(std::max<uint64_t>(S, 64)), gcqf::pow2_at_least(std::max<uint64_t>(S, 64)),
@@ -1015,7 +1017,6 @@ extern "C" int gcq_server_start(gcq_server* s) {
     GCQ_HIP(hipMalloc(&s->gstart, 4ull * 64 * ng));
     GCQ_HIP(hipMalloc(&s->gslot, 4ull * 64 * ng));
     GCQ_HIP(hipMalloc(&s->gbits, 8ull * gwords));
-    GCQ_HIP(hipMalloc(&s->hubs, 2ull * sv::kLanes * sizeof(bk::HubPool)));
     std::vector<sv::GroupCtl> hc(ng);
     for (uint64_t k = 0; k < ng; ++k) {
       hg[k].seen = (uint64_t)(s->gbits + hg[k].seen);
@@ -1074,8 +1075,8 @@ extern "C" int gcq_server_start(gcq_server* s) {
     zero(s->gdepth, 8ull * 64 * hg.size());
     zero(s->gstart, 4ull * 64 * hg.size());
     zero(s->gslot, 4ull * 64 * hg.size());
-    zero(s->hubs, 2ull * sv::kLanes * sizeof(bk::HubPool));
   }
+  zero(s->hubs, 2ull * sv::kLanes * sizeof(bk::HubPool));
   GCQ_HIP(hipGetLastError());
   GCQ_HIP(hipStreamSynchronize(s->stream));
 

@@ -1,6 +1,6 @@
 """GPU parity of long adjacencies in the persistent BFS (one traversal, in
-both claim modes) and in the multi-source BFS (batched and served): a
-vertex with more than 4,096 edges is posted by the wave that settles it and
+both claim modes) and in the batched and served BFS (per-query items and
+multi-source groups): a vertex with more than 4,096 edges is posted by the wave that settles it and
 scanned by the whole grid in 1,024-edge chunks between two barriers of its
 level.  The graph below makes every part of that path run:
 
@@ -82,16 +82,21 @@ def test_single_hubs(gcq, hub_world, kind, threads, mode):
         assert r["reached"] == reached and r["edges_scanned"] == es and r["levels"] == int(ref.max()) + 1
 
 
+@pytest.mark.parametrize("ms", [0, 1], ids=["per_query", "multi_source"])
 @pytest.mark.parametrize("threads", [4096, 65536])
 @pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["gwfq", "glfq", "wfq", "sfq"])
-def test_batch_hubs(gcq, hub_world, kind, threads):
+def test_batch_hubs(gcq, hub_world, kind, threads, ms):
     from gcq_amd import bfs
     g, (off, idx), sources, refs = hub_world
-    print(f"[hubs batch] kind={kind} threads={threads}", flush=True)
+    print(f"[hubs batch] kind={kind} threads={threads} multi_source={ms}", flush=True)
+    if not ms:
+        # per-query items take a frontier of the queries' summed vertices (G-WFQ rings stop at 2^25):
+        # the centre (2,100 hubs at level 1), the second-level hub and the 4,097-edge hub
+        sources = [sources[0], sources[5], sources[2]]
     with bfs.Batch([g]) as b:
         for use_graph in (0, 1, 1):
             outs, per, res = b.run([0] * len(sources), sources, depths=True, kind=kind, threads=threads,
-                                   block=256, use_graph=use_graph, multi_source=1)
+                                   block=256, use_graph=use_graph, multi_source=ms)
             for i, s in enumerate(sources):
                 ref, reached, es = refs[s]
                 bad = np.flatnonzero(outs[i] != ref)
@@ -101,8 +106,9 @@ def test_batch_hubs(gcq, hub_world, kind, threads):
             assert res["levels"] == max(p["levels"] for p in per)
 
 
+@pytest.mark.parametrize("ms", [0, 1], ids=["per_query", "multi_source"])
 @pytest.mark.parametrize("kind", [1, 3], ids=["glfq", "sfq"])
-def test_served_hubs(gcq, orc, hub_world, kind):
+def test_served_hubs(gcq, orc, hub_world, kind, ms):
     # The hub graph beside a small R-MAT graph in the other lane; queries
     # arrive while others run, so members join the hub graph's group at
     # different levels.
@@ -117,10 +123,10 @@ def test_served_hubs(gcq, orc, hub_world, kind):
         gi = 0 if i % 4 else 1
         src = sources[i % len(sources)] if gi == 0 else int(rng.integers(0, 1024))
         rows.append(dict(type=0, graph=gi, src=src, keep=1, id=i, arrival_ms=float(t[i])))
-    print(f"[hubs serve] kind={kind}", flush=True)
+    print(f"[hubs serve] kind={kind} multi_source={ms}", flush=True)
     with serve.Server([g, small], scene, (1920, 1080), kind=kind, budget_bytes=1 << 30, keep_bytes=2 << 30,
                       bfs_slots=64, bfs_threads=16384, rt_threads=4096, frontier_log2=24, small_log2=11,
-                      small_threads=4096, multi_source=1) as srv:
+                      small_threads=4096, multi_source=ms) as srv:
         comps = srv.run_trace(rows)
         soff, sidx = small.csr()
         for r, c in zip(rows, comps):
