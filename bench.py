@@ -292,7 +292,7 @@ def profile_run(a):
         with rt.Scene.preset(rt.RANDOM, 64, 3) as scene:
             for _ in range(a.runs):
                 with serve.Server(graphs, scene, (1920, 1080), bfs_threads=245760, block=512, rt_threads=16384,
-                                  frontier_log2=28, small_log2=18, small_threads=8192, multi_source=1) as srv:
+                                  frontier_log2=28, small_log2=18, small_threads=32768, multi_source=1) as srv:
                     res = serve.latency_summary(srv.run_trace(rows))
         out.update(res)
         for gr in graphs:
@@ -453,58 +453,63 @@ def secondary(g, with_cpu):
     pick = lambda gi, i: graphs[gi].pick_source(11, i)  # noqa: E731
     for gr in graphs:
         gr.bfs(0, kind=g.GLFQ, threads=32768, depths=False)
-    lam = 150.0
-    rows = serve.poisson_trace(300, lam, nvs, frame, rt_fraction=0.1, seed=5, graph_sources=pick)
+    # Poisson traces of 600 requests at four rates, up to and past the plain
+    # path's capacity, served and plain; then one 300-request burst each.
+    rates = (150.0, 300.0, 450.0, 600.0)
+    traces = {lam: serve.poisson_trace(600, lam, nvs, frame, rt_fraction=0.1, seed=5, graph_sources=pick)
+              for lam in rates}
     burst = serve.poisson_trace(300, 1e9, nvs, frame, rt_fraction=0.1, seed=6, graph_sources=pick)
     # two BFS lanes: graphs of at most 2^18 vertices (scales 14..18) get
-    # 8,192 of the BFS threads and their own level barrier; tiles 16,384
-    # threads (a sweep of splits on MI355X: this one keeps the one-lane p99
-    # and cuts p50 about 10x); queries on one graph share bit-parallel groups
+    # 32,768 of the BFS threads and their own level barrier; tiles 16,384
+    # threads; queries on one graph share bit-parallel groups (a sweep of
+    # splits on MI355X: this one has the lowest p50 at every rate and the
+    # highest burst capacity)
     scfg = dict(bfs_threads=245760, block=512, rt_threads=16384, frontier_log2=28, small_log2=18,
-                small_threads=8192, multi_source=1)
-    with serve.Server(graphs, scene, frame, **scfg) as srv:
-        comps = srv.run_trace(rows)
-        s_trace = serve.latency_summary(comps)
-        by_lane = {lane: serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 0 and
-                                               (nvs[r["graph"]] <= 1 << 18) == (lane == "small")])
-                   for lane in ("small", "big")}
-        by_lane["tiles"] = serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 1])
+                small_threads=32768, multi_source=1)
+    pcfg = dict(bfs_threads=65536, rt_threads_per_tile=8192)
+    by_rate = {}
+    for lam in rates:
+        with serve.Server(graphs, scene, frame, **scfg) as srv:
+            comps = srv.run_trace(traces[lam])
+        s_tr = serve.latency_summary(comps)
+        p_tr = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, traces[lam], **pcfg))
+        by_rate[lam] = (s_tr, p_tr, comps)
+    rows = traces[rates[0]]
+    comps = by_rate[rates[0]][2]
+    by_lane = {lane: serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 0 and
+                                           (nvs[r["graph"]] <= 1 << 18) == (lane == "small")])
+               for lane in ("small", "big")}
+    by_lane["tiles"] = serve.latency_summary([c for c, r in zip(comps, rows) if r["type"] == 1])
     with serve.Server(graphs, scene, frame, **scfg) as srv:
         s_burst = serve.latency_summary(srv.run_trace(burst))
-    # the same trace without groups, with two lanes and with one
+    p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, **pcfg))
+    # the same lowest-rate trace and burst without groups (per-query items)
     per_q = dict(scfg, multi_source=0)
     with serve.Server(graphs, scene, frame, **per_q) as srv:
         s_perq = serve.latency_summary(srv.run_trace(rows))
     with serve.Server(graphs, scene, frame, **per_q) as srv:
         s_perq_burst = serve.latency_summary(srv.run_trace(burst))
-    one = {k: v for k, v in per_q.items() if k not in ("small_log2", "small_threads")}
-    with serve.Server(graphs, scene, frame, **one) as srv:
-        s_one = serve.latency_summary(srv.run_trace(rows))
-    pcfg = dict(bfs_threads=65536, rt_threads_per_tile=8192)
-    p_trace = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows, **pcfg))
-    p_burst = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, burst, **pcfg))
-    # a second rate, near the plain path's burst capacity, where its requests queue
-    lam2 = 250.0
-    rows2 = serve.poisson_trace(300, lam2, nvs, frame, rt_fraction=0.1, seed=7, graph_sources=pick)
-    with serve.Server(graphs, scene, frame, **scfg) as srv:
-        s_trace2 = serve.latency_summary(srv.run_trace(rows2))
-    p_trace2 = serve.latency_summary(serve.plain_run_trace(graphs, scene, frame, rows2, **pcfg))
     rnd = lambda d: {k: round(v, 3) if isinstance(v, float) else v for k, v in d.items()}  # noqa: E731
+    trip = lambda d: [round(d["p50_ms"], 2), round(d["p99_ms"], 2), round(d["throughput_rps"], 1)]  # noqa: E731
+    s_top, p_top, _ = by_rate[rates[-1]]
     out["served"] = {
-        "workload": f"Poisson {lam:.0f} req/s, 300 requests: 90% BFS on R-MAT scales 14..22, 10% 256x256 tiles "
-                    "of the 1920x1080 64-sphere scene (4 bounces), 64 GB budget, arrival seed 5",
-        "served": rnd(s_trace), "plain": rnd(p_trace),
+        "workload": "Poisson arrivals at 150 / 300 / 450 / 600 requests/s, 600 requests each (and a 300-request "
+                    "burst): 90% BFS on R-MAT scales 14..22, 10% 256x256 tiles of the 1920x1080 64-sphere scene "
+                    "(4 bounces), 64 GB budget, arrival seed 5 (burst seed 6)",
+        "served": rnd(by_rate[rates[0]][0]), "plain": rnd(by_rate[rates[0]][1]),
+        "by_rate": {f"{lam:.0f}": {"served_p50_p99_rps": trip(by_rate[lam][0]),
+                                   "plain_p50_p99_rps": trip(by_rate[lam][1])} for lam in rates},
         "served_by_class": {k: rnd(v) for k, v in by_lane.items()},
         "served_per_query_items": rnd(s_perq), "burst_served_per_query_items": rnd(s_perq_burst),
-        "served_one_lane_per_query_items": rnd(s_one),
         "burst_served": rnd(s_burst), "burst_plain": rnd(p_burst),
-        f"served_at_{lam2:.0f}_per_s": rnd(s_trace2), f"plain_at_{lam2:.0f}_per_s": rnd(p_trace2),
         "capacity_vs_plain": round(s_burst["throughput_rps"] / p_burst["throughput_rps"], 2),
         "config": dict(scfg, plain_bfs_threads=pcfg["bfs_threads"],
                        plain_rt_threads_per_tile=pcfg["rt_threads_per_tile"])}
     for gr in graphs:
         gr.close()
-    _progress(f"served p99 {s_trace['p99_ms']:.1f} ms vs plain {p_trace['p99_ms']:.1f} ms")
+    _progress(f"served at {rates[-1]:.0f}/s p50 {s_top['p50_ms']:.1f} / p99 {s_top['p99_ms']:.1f} ms vs plain "
+              f"{p_top['p50_ms']:.1f} / {p_top['p99_ms']:.1f} ms; burst capacity "
+              f"{out['served']['capacity_vs_plain']}x plain")
 
     # --- ray tracing
     rcfg = dict(width=1920, height=1080, tile_w=256, tile_h=256, threads_per_tile=8192, block=512)

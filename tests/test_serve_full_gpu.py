@@ -2,7 +2,7 @@
 14..22 (edge factor 16, seed 11), the 1920x1080 64-sphere scene, a 64 GB
 request budget and the benchmark's server configuration (a 2^28-slot
 frontier ring, 245,760 BFS + 16,384 RT threads in blocks of 512), with one
-BFS lane and with 8,192 of the BFS threads in a second lane for the graphs
+BFS lane and with 32,768 of the BFS threads in a second lane for the graphs
 of at most 2^18 vertices, with and without bit-parallel groups for the
 queries on one graph.  About 60
 requests, 90% BFS queries on sources in each graph's giant component, 10%
@@ -19,8 +19,8 @@ FRAME = (1920, 1080)
 SCALES = list(range(14, 23))
 SERVER = dict(bfs_threads=245760, block=512, rt_threads=16384, frontier_log2=28, budget_bytes=64 << 30,
               keep_bytes=4 << 30)
-LANES = {1: {}, 2: dict(small_log2=18, small_threads=8192), "ms": dict(small_log2=18, small_threads=8192,
-                                                                       multi_source=1)}
+LANES = {1: {}, 2: dict(small_log2=18, small_threads=32768),
+         "ms": dict(small_log2=18, small_threads=32768, multi_source=1)}
 
 
 @pytest.fixture(scope="module")
