@@ -268,7 +268,7 @@ def profile_run(a):
         with bfs.Batch(graphs) as b:
             for _ in range(a.runs):
                 # the configuration the autotuner picked for the bench line on MI355X
-                _, _, res = b.run(gids, srcs, threads=65536, block=256, pop_width=16, flush_at=64, capacity_log2=26,
+                _, _, res = b.run(gids, srcs, threads=65536, block=128, pop_width=32, flush_at=64, capacity_log2=0,
                                   multi_source=1, use_graph=0)
         out.update(res)
         for gr in graphs:
