@@ -65,14 +65,17 @@ __global__ void k_batch(Frontiers<F> fq, Dev d, Ctl* ctl, HubPool* hub) {
       if (F::kQueues == 3) fq.idle(L).rewind();
       hub_reset(&hub[(L + 1) & 1]);
     };
-    if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
-          if (gcq::ld_rlx(&hp->word) == 0ull) level_end();  // else the level ends after its hub scan
-        }))
+    uint32_t hubs_posted = 0;
+    if (!grid_barrier(&ctl->bar, gridDim.x, [&]() -> uint32_t {
+          if (gcq::ld_rlx(&hp->word) != 0ull) return 1u;  // the level ends after its hub scan
+          level_end();
+          return 0u;
+        }, &hubs_posted))
       break;
     // Drop this CU's and XCD's cached lines so the cached visited probes of
     // the next level see at least every bit set up to this barrier.
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (gcq::ld_rlx(&hp->word) != 0ull) {
+    if (hubs_posted) {
       const uint32_t dh = wave_sum32(hubs<F, false>(fq.out(L), d, hp, L, tid, (uint32_t)(nthreads / 64), st));
       if (lane_id() == 0 && dh) gcq::faa_rlx(&ctl->next[L % 3], dh);
       if (!grid_barrier(&ctl->bar, gridDim.x, level_end)) break;
@@ -118,12 +121,15 @@ __global__ void k_batch_ms(Frontiers<F> fq, MsDev d, Ctl* ctl, HubPool* hub) {
       if (F::kQueues == 3) fq.idle(L).rewind();
       hub_reset(&hub[(L + 1) & 1]);
     };
-    if (!grid_barrier(&ctl->bar, gridDim.x, [&]() {
-          if (gcq::ld_rlx(&hp->word) == 0ull) level_end();  // else the level ends after its hub scan
-        }))
+    uint32_t hubs_posted = 0;
+    if (!grid_barrier(&ctl->bar, gridDim.x, [&]() -> uint32_t {
+          if (gcq::ld_rlx(&hp->word) != 0ull) return 1u;  // the level ends after its hub scan
+          level_end();
+          return 0u;
+        }, &hubs_posted))
       break;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (gcq::ld_rlx(&hp->word) != 0ull) {
+    if (hubs_posted) {
       const uint32_t dh = wave_sum32(ms_hubs(fq.out(L), d, hp, L, tid, (uint32_t)(nthreads / 64), st));
       if (lane_id() == 0 && dh) gcq::faa_rlx(&ctl->next[L % 3], dh);
       if (!grid_barrier(&ctl->bar, gridDim.x, level_end)) break;

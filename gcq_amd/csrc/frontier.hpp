@@ -281,23 +281,32 @@ __device__ __forceinline__ void flush(const F& qout, uint32_t tid, uint32_t* sta
 // Barrier words, each on its own 128-B line.
 struct Barrier {
   uint32_t count, p0[31];
-  uint32_t gen, p1[31];
+  uint32_t gen, flag, p1[30];  // flag: what the last block's last() returned, stored before gen
   uint32_t abort, p2[31];
 };
 
 // Sense-reversing grid barrier over co-resident blocks.  The last block to
-// arrive runs `last()` before releasing the others.  Returns false once any
-// block has aborted (spin limit), so every block leaves its loop.
+// arrive runs `last()` before releasing the others; when last() returns a
+// value, every thread of every block gets it in *flag (one read per block,
+// after the release).  Returns false once any block has aborted (spin
+// limit), so every block leaves its loop.
 template <class Last>
-__device__ bool grid_barrier(Barrier* b, uint32_t nblocks, Last last) {
+__device__ bool grid_barrier(Barrier* b, uint32_t nblocks, Last last, uint32_t* flag = nullptr) {
+  __shared__ uint32_t s_flag, s_ok;
   gcq::vm_wait();
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint32_t g = gcq::ld_rlx(&b->gen);
     gcq::vm_wait();  // generation read before arriving
     const uint32_t a = gcq::faa_rlx(&b->count, 1u);
+    uint32_t f = 0;
     if (a == nblocks - 1) {
-      last();
+      if constexpr (std::is_void_v<decltype(last())>) {
+        last();
+      } else {
+        f = last();
+        gcq::st_rlx(&b->flag, f);
+      }
       gcq::st_rlx(&b->count, 0u);
       gcq::vm_wait();
       gcq::st_rlx(&b->gen, g + 1u);
@@ -311,10 +320,17 @@ __device__ bool grid_barrier(Barrier* b, uint32_t nblocks, Last last) {
         }
         __builtin_amdgcn_s_sleep(2);
       }
+      if constexpr (!std::is_void_v<decltype(last())>) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the flag stored before the release
+        f = gcq::ld_rlx(&b->flag);
+      }
     }
+    s_flag = f;
+    s_ok = gcq::ld_rlx(&b->abort) == 0u;  // one read per block, not one per wave
   }
   __syncthreads();
-  return gcq::ld_rlx(&b->abort) == 0u;
+  if (flag) *flag = s_flag;
+  return s_ok != 0u;
 }
 
 // ---------------------------------------------------------------- hub vertices

@@ -417,12 +417,15 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
     };
     // A level that posted long adjacencies ends after the lane's scan of
     // them, at a second barrier.
-    bool ok = grid_barrier(&lc->bar, nblocks, [&]() {
-      if (gcq::ld_rlx(&hp->word) == 0ull) level_end();
-    });
+    uint32_t hubs_posted = 0;
+    bool ok = grid_barrier(&lc->bar, nblocks, [&]() -> uint32_t {
+      if (gcq::ld_rlx(&hp->word) != 0ull) return 1u;
+      level_end();
+      return 0u;
+    }, &hubs_posted);
     if (ok) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (gcq::ld_rlx(&hp->word) != 0ull) {
+      if (hubs_posted) {
         if (d.ms) (void)bk::ms_hubs(fq.out(L), d.md, hp, L, tid, (uint32_t)(nthreads / 64), stage);
         else (void)bk::hubs<F, false>(fq.out(L), d.bd, hp, L, tid, (uint32_t)(nthreads / 64), stage);
         ok = grid_barrier(&lc->bar, nblocks, level_end);
