@@ -151,52 +151,105 @@ struct FLfq {
     return true;
   }
 };
+// SFQ's ticket/phase protocol on slots that carry a 32-bit item: the slot's
+// 64-bit word holds the phase in [63:32] and the item in [31:0], so one load
+// gives both and one store publishes both.  Ticket t's slot is t mod qlen,
+// its lap l = t / qlen; phase 2l (mod 2^32) is empty for the lap and 2l + 1
+// full.  Against the queue's general form (item and 64-bit phase in two
+// words) a pop saves the item's load and a push the wait between its item
+// and phase stores.  The ring is the queue's own (its slots' phase words
+// start at 0 = lap 0 empty); the item words stay unused.
 struct FSfq {
   static constexpr int kQueues = 2;
   static constexpr bool kCounted = true;
   static constexpr bool kHeadFromDrain = true;
   gcq::sfq::Queue* q;
+  __device__ __forceinline__ uint64_t* word(uint64_t t) const {
+    return &gcq::sfq::slots(q)[t & (q->g.qlen - 1)].id;
+  }
+  __device__ __forceinline__ uint32_t lap_phase(uint64_t t) const { return (uint32_t)(2ull * (t >> q->g.qlen_log2)); }
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
   // Counted pops: the slot of a ticket below the level's end carries its
-  // item and the "full" phase id; the pop reads the item and hands the slot
-  // to the next lap, as a blocking dequeue does after its wait.
+  // item and the full phase; the pop takes the item from the word it read
+  // and hands the slot to the next lap, as a blocking dequeue does after
+  // its wait.
   __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::sfq::tail(q)); }
   __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
     const uint64_t T = gcq::wave_faa(gcq::sfq::head(q), (uint64_t)1);
     if (T >= end) return -1;
     return take(tid, T, end, peek(T), v);
   }
-  __device__ __forceinline__ uint64_t peek(uint64_t T) const {
-    return gcq::ld_rlx(&gcq::sfq::slots(q)[T & (q->g.qlen - 1)].id);
-  }
-  // Ticket T with its slot's phase id as first read: as pop_counted after its FAA.
-  __device__ __forceinline__ int take(uint32_t, uint64_t T, uint64_t end, uint64_t first_id, uint32_t* v) const {
-    using namespace gcq::sfq;
+  __device__ __forceinline__ uint64_t peek(uint64_t T) const { return gcq::ld_rlx(word(T)); }
+  // Ticket T with its slot's word as first read: as pop_counted after its FAA.
+  __device__ __forceinline__ int take(uint32_t, uint64_t T, uint64_t end, uint64_t first, uint32_t* v) const {
     if (T >= end) return -1;
-    Slot* s = &slots(q)[T & (q->g.qlen - 1)];
-    const uint64_t id = 2ull * (T >> q->g.qlen_log2) + 1ull;
-    bool ready = first_id == id;
-    for (uint32_t w = 0; !ready && w < gcq::kSlotWaitSpins; ++w) {
+    const uint32_t full = lap_phase(T) + 1u;
+    uint64_t w = first;
+    for (uint32_t k = 0; (uint32_t)(w >> 32) != full && k < gcq::kSlotWaitSpins; ++k) {
       __builtin_amdgcn_s_sleep(1);
-      ready = gcq::ld_rlx(&s->id) == id;
+      w = gcq::ld_rlx(word(T));
     }
-    if (!ready) return 0;
-    const uint64_t x = gcq::ld_rlx(&s->item);
-    gcq::vm_wait();  // item read completes before the slot is handed on
-    gcq::st_rlx(&s->id, id + 1);
-    *v = (uint32_t)x;
+    if ((uint32_t)(w >> 32) != full) return 0;
+    gcq::st_rlx(word(T), (uint64_t)(full + 1u) << 32);  // empty for the next lap
+    *v = (uint32_t)w;
     return 1;
   }
   __device__ __forceinline__ void drained(uint64_t end) const { gcq::st_rlx(gcq::sfq::head(q), end); }
+  // Blocking enqueue (sfqueue_hip.cpp:55-70): a ticket by wave-aggregated
+  // FAA on Tail, a wait (bounded) for the slot's previous lap to be taken,
+  // then item and full phase in one store.
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
-    return gcq::sfq::enqueue_blocking(q, (uint64_t)v, 1ull << 22) == gcq::sfq::kSuccess;
-  }
-  __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
-    uint64_t x = 0;
-    if (!gcq::sfq::dequeue_or_empty(q, &x, 1ull << 20)) return false;
-    *v = (uint32_t)x;
+    const uint64_t t = gcq::wave_faa(gcq::sfq::tail(q), (uint64_t)1);
+    const uint32_t empty = lap_phase(t);
+    for (uint64_t k = 0; (uint32_t)(gcq::ld_rlx(word(t)) >> 32) != empty; ++k) {
+      if (k > (1ull << 22)) return false;
+      gcq::sfq::slot_wait(k, q->g.backoff);
+    }
+    gcq::st_rlx(word(t), ((uint64_t)(empty + 1u) << 32) | v);
     return true;
+  }
+  // Non-blocking pop (host-loop and barrier-free modes, the ray tracer): the
+  // queue's non-blocking dequeue (sfqueue_hip.cpp:109-128) on packed slots.
+  // The wave reads Head once, tests its lanes' tickets, and claims the
+  // prefix whose slots are full with one CAS; it answers EMPTY only after
+  // reading Head >= Tail (head read first), and retries BUSY otherwise.
+  __device__ bool pop(uint32_t, uint32_t* v) const {
+    uint64_t* const hp = gcq::sfq::head(q);
+    for (uint32_t r = 0; r < (1u << 20); ++r) {
+      const uint64_t m = gcq::active_mask();
+      const uint32_t leader = gcq::first_lane(m);
+      uint64_t t0 = 0;
+      if (gcq::lane_id() == leader) t0 = gcq::ld_rlx(hp);
+      t0 = gcq::bcast64(t0, leader);
+      const uint64_t t = t0 + gcq::rank_in(m);
+      const uint64_t w = gcq::ld_rlx(word(t));
+      const bool ready = (uint32_t)(w >> 32) == lap_phase(t) + 1u;
+      const uint64_t not_ready = m & ~__ballot(ready);
+      const uint32_t k = not_ready ? (uint32_t)__builtin_popcountll(m & ((1ull << gcq::first_lane(not_ready)) - 1ull))
+                                   : (uint32_t)__builtin_popcountll(m);
+      uint32_t won = 0;
+      if (gcq::lane_id() == leader && k > 0) {
+        uint64_t exp = t0;
+        won = gcq::cas_rlx(hp, exp, t0 + k) ? 1u : 0u;
+      }
+      won = gcq::bcast32(won, leader);
+      if (won && gcq::rank_in(m) < k) {
+        gcq::st_rlx(word(t), (uint64_t)(lap_phase(t) + 2u) << 32);  // empty for the next lap
+        *v = (uint32_t)w;
+        return true;
+      }
+      if (won) continue;  // lanes past the claimed prefix try again at once
+      uint32_t empty = 0;
+      if (gcq::lane_id() == leader) {
+        const uint64_t h = gcq::ld_rlx(hp);
+        gcq::vm_wait();  // Head is read before Tail: Head (at the Tail read) >= h >= Tail means empty
+        empty = h >= gcq::ld_rlx(gcq::sfq::tail(q)) ? 1u : 0u;
+      }
+      if (gcq::bcast32(empty, leader)) return false;
+      gcq::sfq::backoff(8);
+    }
+    return false;
   }
 };
 // WFQ cells are never reused by the algorithm, and every level burns up to one
