@@ -50,6 +50,13 @@ HBM_PEAK_GBS = 8000.0
 # indirection: a free-index ticket slot read + consume (8 + 8), the payload
 # (8), the published-index slot read + CAS (8 + 8).
 BYTES_PER_OP = 40
+# FP32 operations of one traced ray of the 64-sphere scene (contraction off,
+# so each is one VALU operation): 17 per sphere test (oc 3, b 5, c 7, disc 2)
+# x 64, 13 for the ground plane, about 80 for the hit point, normal, shading
+# and reflected ray.  MI355X's published FP32 vector peak, 157.3 TFLOP/s,
+# counts packed FMAs; plain adds and multiplies issue at a quarter of it.
+RT_FLOP_PER_RAY = 17 * 64 + 13 + 80
+FP32_PEAK_TFLOPS = 157.3
 
 
 def _profile_traffic(name):
@@ -523,6 +530,12 @@ def secondary(g, with_cpu):
                  "rays": res["total_rays"], "kernel_ms_median": round(med, 3),
                  "mrays_per_s": round(res["total_rays"] / med / 1e3, 1),
                  "config": {"threads_per_tile": 8192, "block": 512}}
+    rt_tflops = RT_FLOP_PER_RAY * res["total_rays"] / med / 1e9
+    out["rt"]["roofline"] = {"bound": "fp32", "achieved": round(rt_tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(rt_tflops / FP32_PEAK_TFLOPS, 4),
+                             "flop_per_ray": RT_FLOP_PER_RAY,
+                             "note": "no FMA contraction (bit-exact shading), so plain FP32 adds and multiplies "
+                                     "cap it at a quarter of the packed-FMA peak"}
     # the stream-compaction comparator on the same frame (same image)
     scene.render_compact(width=1920, height=1080, block=1024)
     cms = []

@@ -56,6 +56,12 @@ namespace sv {
 using namespace gcqf;
 
 constexpr uint32_t kMaxAdmit = 32;           // BFS queries installed per BFS barrier
+// An idle BFS lane sleeps 32 x 127 x 64 cycles (about 0.1 ms) per level.
+// Measured on the served mix (Poisson, 20 and 150 requests/s) against no
+// park: small-graph p50 2.3 -> 0.9 ms and tile p50 0.58 -> 0.18 ms at
+// 20/s, overall p50 3.3 -> 2.0 ms at 150/s; burst capacity unchanged.
+// Shorter parks (4, 16) kept more of the interference, 64 gained nothing.
+constexpr uint32_t kIdleParkSleeps = 32;
 constexpr uint64_t kRtIdleLimit = 1ull << 30;  // idle polls of an RT wave before it gives up
 constexpr uint32_t kMaxTilePixels = 1u << 16;  // RT items carry a 16-bit pixel and a 15-bit job generation
 constexpr uint32_t kRtSubmitEvery = 32;        // passes of a busy RT wave between looks at the tile queue
@@ -229,8 +235,10 @@ __device__ void gate(const Dev& d) {
 }
 
 // Install a lane's queued BFS queries (one thread: the last block to reach
-// the barrier that ends the lane's level L).
-__device__ void admit_bfs(const Dev& d, int lane, int32_t L) {
+// the barrier that ends the lane's level L).  True when the lane is idle:
+// nothing admitted here and no query of the lane still running, so its next
+// level has no items.
+__device__ bool admit_bfs(const Dev& d, int lane, int32_t L) {
   Ctl* c = d.c;
   LaneCtl* lc = &c->lane[lane];
   uint32_t n = 0;
@@ -273,6 +281,7 @@ __device__ void admit_bfs(const Dev& d, int lane, int32_t L) {
     gcq::st_rlx(&lc->quit, 1u);
     gcq::st_rlx(&c->exit, 1u);  // for the gate and the RT waves
   }
+  return n == 0 && gcq::ld_rlx(&lc->active) == 0;
 }
 
 // Install one queued tile job, if any (lane 0 of an RT wave); true if one was.  The job's
@@ -410,25 +419,25 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
       (void)bk::expand<F, false>(fq.in(L), fq.out(L), d.bd, L, tid, stage,
                                  F::kHeadFromDrain ? prev_end[L & 1] : kNoHead, nthreads / 64, &end, hp);
     prev_end[L & 1] = end;
-    auto level_end = [&]() {
+    auto level_end = [&]() -> bool {
       if (F::kQueues == 3) fq.idle(L).rewind();
       bk::hub_reset(&d.hub[2 * lane + ((L + 1) & 1)]);
-      admit_bfs(d, lane, L);
+      return admit_bfs(d, lane, L);
     };
     // A level that posted long adjacencies ends after the lane's scan of
-    // them, at a second barrier.
-    uint32_t hubs_posted = 0;
+    // them, at a second barrier.  flag: 1 long adjacencies posted, 2 the
+    // lane is idle.
+    uint32_t flag = 0;
     bool ok = grid_barrier(&lc->bar, nblocks, [&]() -> uint32_t {
       if (gcq::ld_rlx(&hp->word) != 0ull) return 1u;
-      level_end();
-      return 0u;
-    }, &hubs_posted);
+      return level_end() ? 2u : 0u;
+    }, &flag);
     if (ok) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (hubs_posted) {
+      if (flag == 1u) {
         if (d.ms) (void)bk::ms_hubs(fq.out(L), d.md, hp, L, tid, (uint32_t)(nthreads / 64), stage);
         else (void)bk::hubs<F, false>(fq.out(L), d.bd, hp, L, tid, (uint32_t)(nthreads / 64), stage);
-        ok = grid_barrier(&lc->bar, nblocks, level_end);
+        ok = grid_barrier(&lc->bar, nblocks, [&]() { (void)level_end(); });
       }
     }
     if (!ok) {
@@ -441,6 +450,12 @@ __device__ void bfs_part(const Frontiers<F>& fq, const Dev& d, int lane, uint32_
       if (d.ms) done_scan_ms(d, lane, L);
       else done_scan(d, lane, L);
     }
+    // An idle lane parks before its next (empty) level, so that its blocks'
+    // barrier atomics, release polls and cache invalidations stay off the
+    // memory system while the other lane and the tile waves run; a query
+    // then waits at most one park for admission.
+    if (flag == 2u)
+      for (uint32_t i = 0; i < kIdleParkSleeps; ++i) __builtin_amdgcn_s_sleep(127);
   }
   if (tid == 0) lc->levels = L + 1;
 }
