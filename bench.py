@@ -53,9 +53,13 @@ BYTES_PER_OP = 40
 # FP32 operations of one traced ray of the 64-sphere scene (contraction off,
 # so each is one VALU operation): 17 per sphere test (oc 3, b 5, c 7, disc 2)
 # x 64, 13 for the ground plane, about 80 for the hit point, normal, shading
-# and reflected ray.  MI355X's published FP32 vector peak, 157.3 TFLOP/s,
-# counts packed FMAs; plain adds and multiplies issue at a quarter of it.
+# and reflected ray.  A primary ray starts at the camera, whose per-object
+# terms (oc and c per sphere, the plane's offset) the scene computes once: 7
+# per sphere test and 7 for the plane remain.  MI355X's published FP32
+# vector peak, 157.3 TFLOP/s, counts packed FMAs; plain adds and multiplies
+# issue at a quarter of it.
 RT_FLOP_PER_RAY = 17 * 64 + 13 + 80
+RT_FLOP_PER_PRIMARY = 7 * 64 + 7 + 80
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -530,10 +534,12 @@ def secondary(g, with_cpu):
                  "rays": res["total_rays"], "kernel_ms_median": round(med, 3),
                  "mrays_per_s": round(res["total_rays"] / med / 1e3, 1),
                  "config": {"threads_per_tile": 8192, "block": 512}}
-    rt_tflops = RT_FLOP_PER_RAY * res["total_rays"] / med / 1e9
+    n_primary = 1920 * 1080
+    rt_flop = RT_FLOP_PER_PRIMARY * n_primary + RT_FLOP_PER_RAY * (res["total_rays"] - n_primary)
+    rt_tflops = rt_flop / med / 1e9
     out["rt"]["roofline"] = {"bound": "fp32", "achieved": round(rt_tflops, 2), "peak": FP32_PEAK_TFLOPS,
                              "unit": "TFLOP/s", "frac": round(rt_tflops / FP32_PEAK_TFLOPS, 4),
-                             "flop_per_ray": RT_FLOP_PER_RAY,
+                             "flop_per_ray": {"primary": RT_FLOP_PER_PRIMARY, "bounce": RT_FLOP_PER_RAY},
                              "note": "no FMA contraction (bit-exact shading), so plain FP32 adds and multiplies "
                                      "cap it at a quarter of the packed-FMA peak"}
     # the stream-compaction comparator on the same frame (same image)

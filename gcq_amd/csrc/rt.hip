@@ -67,18 +67,22 @@ __global__ void k_render(const F* __restrict__ queues, Scene sc, Job jb) {
   uint32_t next_primary = wt;
   uint64_t traced = 0;
   uint32_t idle = 0;
+  bool held = false;  // this lane's ray continues here (its primaries are done)
+  uint32_t i = 0;
+  Ray r;
   for (;;) {
-    bool have = false;
-    uint32_t i = 0;
-    Ray r;
-    if (next_primary < npx) {
-      i = next_primary;
-      next_primary += T;
-      r = primary(sc, jb.W, jb.H, jb.x0 + lx0 + (int32_t)(i % (uint32_t)tw), jb.y0 + ly0 + (int32_t)(i / (uint32_t)tw));
-      have = true;
-    } else if (q.pop(wt, &i)) {
-      r = load_ray(&jb.recs[(ly0 + (int32_t)(i / (uint32_t)tw)) * jb.w + lx0 + (int32_t)(i % (uint32_t)tw)]);
-      have = true;
+    bool have = held;
+    if (!held) {
+      if (next_primary < npx) {
+        i = next_primary;
+        next_primary += T;
+        r = primary(sc, jb.W, jb.H, jb.x0 + lx0 + (int32_t)(i % (uint32_t)tw),
+                    jb.y0 + ly0 + (int32_t)(i / (uint32_t)tw));
+        have = true;
+      } else if (q.pop(wt, &i)) {
+        r = load_ray(&jb.recs[(ly0 + (int32_t)(i / (uint32_t)tw)) * jb.w + lx0 + (int32_t)(i % (uint32_t)tw)]);
+        have = true;
+      }
     }
     if (__ballot(have) == 0) {
       // No primaries left for this wave and the queue looked empty: done
@@ -100,17 +104,24 @@ __global__ void k_render(const F* __restrict__ queues, Scene sc, Job jb) {
       ++traced;
       const uint32_t rp = (uint32_t)((ly0 + (int32_t)(i / (uint32_t)tw)) * jb.w + lx0 + (int32_t)(i % (uint32_t)tw));
       if (shade(sc, jb.max_depth, light, r)) {
-        store_ray(&jb.recs[rp], r);
-        gcq::vm_wait();  // record complete before its index is published
-        if (!q.push(wt, i)) {
-          gcq::or_rlx(jb.err, 1u);
-          ended = 1;
+        // While the lane still has primaries, the bounce ray goes through
+        // the tile's queue to whichever lane is free; once they are done
+        // the lane traces it on itself, without the queue's round trip.
+        held = next_primary >= npx;
+        if (!held) {
+          store_ray(&jb.recs[rp], r);
+          gcq::vm_wait();  // record complete before its index is published
+          if (!q.push(wt, i)) {
+            gcq::or_rlx(jb.err, 1u);
+            ended = 1;
+          }
         }
       } else {
         jb.img[3 * rp + 0] = r.acc.x;
         jb.img[3 * rp + 1] = r.acc.y;
         jb.img[3 * rp + 2] = r.acc.z;
         ended = 1;
+        held = false;
       }
     }
     const uint32_t e = gcq::wave_sum((uint64_t)ended);
@@ -148,6 +159,15 @@ __global__ void k_init_tiles_ymc(char* pool, uint64_t stride, char* hpool, uint6
   for (uint32_t t = 0; t < ntiles; ++t)
     gcq::ymc::init_slice((gcq::ymc::Queue*)(pool + t * stride), (gcq::ymc::Handle*)(hpool + t * hstride), cells,
                          nthreads, 16, gid, st);
+}
+
+// The camera origin's object-test terms (PrimaryTerms), one thread per
+// object.
+__global__ void k_primary_terms(const gcq_rt_sphere* sph, int32_t ns, const gcq_rt_plane* pl, int32_t np, V3 o,
+                                PrimaryTerms* pre) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < ns) pre[i] = sphere_terms(o, sph[i]);
+  else if (i < ns + np) pre[i] = plane_terms(o, pl[i - ns]);
 }
 
 }  // namespace rtk
@@ -290,6 +310,15 @@ int rtk::scene_upload(gcq_rt_scene* s) {
     GCQ_HIP(hipMemcpy(s->d_sph, s->sph.data(), sizeof(gcq_rt_sphere) * s->sph.size(), hipMemcpyHostToDevice));
   if (!s->pl.empty())
     GCQ_HIP(hipMemcpy(s->d_pl, s->pl.data(), sizeof(gcq_rt_plane) * s->pl.size(), hipMemcpyHostToDevice));
+  const int32_t nobj = (int32_t)(s->sph.size() + s->pl.size());
+  GCQ_HIP(hipMalloc(&s->d_pre, sizeof(rtk::PrimaryTerms) * std::max<int32_t>(1, nobj)));
+  if (nobj) {
+    const float* o = s->desc.cam_o;
+    hipLaunchKernelGGL(rtk::k_primary_terms, dim3((nobj + 255) / 256), dim3(256), 0, s->stream, s->d_sph,
+                       (int32_t)s->sph.size(), s->d_pl, (int32_t)s->pl.size(), rtk::v3(o[0], o[1], o[2]), s->d_pre);
+    GCQ_HIP(hipGetLastError());
+    GCQ_HIP(hipStreamSynchronize(s->stream));
+  }
   return GCQ_OK;
 }
 
@@ -297,6 +326,7 @@ rtk::Scene rtk::scene_view(const gcq_rt_scene* s) {
   rtk::Scene sc;
   sc.sph = s->d_sph;
   sc.pl = s->d_pl;
+  sc.pre = s->d_pre;
   sc.ns = (int32_t)s->sph.size();
   sc.np = (int32_t)s->pl.size();
   const gcq_rt_scene_desc& d = s->desc;
@@ -432,6 +462,7 @@ extern "C" int gcq_rt_scene_destroy(gcq_rt_scene* s) {
   free_ctx(s);
   rtk::free_compact(s);
   if (s->d_sph) (void)hipFree(s->d_sph);
+  if (s->d_pre) (void)hipFree(s->d_pre);
   if (s->d_pl) (void)hipFree(s->d_pl);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;

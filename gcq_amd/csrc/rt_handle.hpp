@@ -12,6 +12,7 @@ struct gcq_rt_scene {
   // device side (lazy)
   gcq_rt_sphere* d_sph = nullptr;
   gcq_rt_plane* d_pl = nullptr;
+  rtk::PrimaryTerms* d_pre = nullptr;  // the camera origin's terms (scene_upload)
   hipStream_t stream = nullptr;
   // render context
   int kind = -1;

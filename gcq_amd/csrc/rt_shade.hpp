@@ -79,9 +79,19 @@ __device__ __forceinline__ Ray load_ray(const RayRec* r) {
   return x;
 }
 
+// Terms of the object tests that depend only on the ray origin, computed
+// once per scene for the camera origin that every primary ray starts from:
+// per sphere oc = o - c and dot(oc, oc) - r * r, per plane dot(n, o) + d
+// (in ocx).  They are the same float operations the tests perform, so a
+// primary ray's hits are bit-identical with or without them.
+struct PrimaryTerms {
+  float ocx, ocy, ocz, c;
+};
+
 struct Scene {
   const gcq_rt_sphere* sph;
   const gcq_rt_plane* pl;
+  const PrimaryTerms* pre;  // ns sphere terms, then np plane terms; null: none
   int32_t ns, np;
   V3 cam_o, cam_f, cam_r, cam_u;  // f, r, u already normalised on the host
   V3 light_raw, ambient;
@@ -93,26 +103,99 @@ struct Hit {
   float t;
   int32_t kind, idx;  // kind 0 none, 1 sphere, 2 plane
 };
+// The scene's spheres and planes are uploaded once and no kernel writes
+// them, so the tests read them through the constant address space: every
+// lane tests the same object, and the compiler turns those uniform reads
+// into scalar loads issued ahead of the arithmetic, where a global pointer
+// gave one vector load and one wait per object.
+typedef const __attribute__((address_space(4))) gcq_rt_sphere* ConstSphere;
+typedef const __attribute__((address_space(4))) gcq_rt_plane* ConstPlane;
+// One sphere test (intersect_sphere, rt_final.cpp:368-385) against the
+// best hit so far.
+__device__ __forceinline__ void sphere_test(Hit& h, int32_t i, V3 o, V3 d, float cx, float cy, float cz, float r) {
+  const V3 oc = o - v3(cx, cy, cz);
+  const float b = dot(oc, d);
+  const float c = dot(oc, oc) - r * r;
+  const float disc = b * b - c;
+  if (disc >= 0.f) {
+    const float t = -b - sqrtf(disc);
+    if (t > 1e-4f && t < h.t) h = Hit{t, 1, i};
+  }
+}
 __device__ __forceinline__ Hit nearest(const Scene& sc, V3 o, V3 d) {
   Hit h{1e30f, 0, -1};
-  for (int32_t i = 0; i < sc.ns; ++i) {
-    const gcq_rt_sphere& s = sc.sph[i];
-    const V3 oc = o - ld3(s.c);
-    const float b = dot(oc, d);
-    const float c = dot(oc, oc) - s.r * s.r;
-    const float disc = b * b - c;
-    if (disc >= 0.f) {
-      const float t = -b - sqrtf(disc);
-      if (t > 1e-4f && t < h.t) h = Hit{t, 1, i};
+  const ConstSphere sph = (ConstSphere)sc.sph;
+  const ConstPlane pl = (ConstPlane)sc.pl;
+  // Spheres in groups of 8: the group's centres and radii are loaded first,
+  // so their scalar loads are in flight together rather than one wait per
+  // sphere; the tests then run in sphere order as before.
+  constexpr int32_t kGroup = 8;
+  int32_t i = 0;
+  for (; i + kGroup <= sc.ns; i += kGroup) {
+    float cx[kGroup], cy[kGroup], cz[kGroup], rr[kGroup];
+#pragma unroll
+    for (int32_t k = 0; k < kGroup; ++k) {
+      cx[k] = sph[i + k].c[0];
+      cy[k] = sph[i + k].c[1];
+      cz[k] = sph[i + k].c[2];
+      rr[k] = sph[i + k].r;
     }
+#pragma unroll
+    for (int32_t k = 0; k < kGroup; ++k) sphere_test(h, i + k, o, d, cx[k], cy[k], cz[k], rr[k]);
   }
-  for (int32_t i = 0; i < sc.np; ++i) {
-    const gcq_rt_plane& p = sc.pl[i];
-    const V3 n = ld3(p.n);
+  for (; i < sc.ns; ++i) sphere_test(h, i, o, d, sph[i].c[0], sph[i].c[1], sph[i].c[2], sph[i].r);
+  for (int32_t j = 0; j < sc.np; ++j) {
+    const auto& p = pl[j];
+    const V3 n = v3(p.n[0], p.n[1], p.n[2]);
     const float denom = dot(n, d);
     if (fabsf(denom) > 1e-5f) {
       const float t = -(dot(n, o) + p.d) / denom;
-      if (t > 1e-4f && t < h.t) h = Hit{t, 2, i};
+      if (t > 1e-4f && t < h.t) h = Hit{t, 2, j};
+    }
+  }
+  return h;
+}
+
+__device__ __forceinline__ PrimaryTerms sphere_terms(V3 o, const gcq_rt_sphere& s) {
+  const V3 oc = o - v3(s.c[0], s.c[1], s.c[2]);
+  return PrimaryTerms{oc.x, oc.y, oc.z, dot(oc, oc) - s.r * s.r};
+}
+__device__ __forceinline__ PrimaryTerms plane_terms(V3 o, const gcq_rt_plane& p) {
+  return PrimaryTerms{dot(v3(p.n[0], p.n[1], p.n[2]), o) + p.d, 0.f, 0.f, 0.f};
+}
+
+// nearest() for a ray from the camera origin, from the scene's primary
+// terms: per sphere b = dot(oc, d) and disc = b * b - c remain (7 of the
+// 17 operations), per plane the denominator.
+typedef const __attribute__((address_space(4))) PrimaryTerms* ConstTerms;
+__device__ __forceinline__ void sphere_test_pre(Hit& h, int32_t i, V3 d, PrimaryTerms q) {
+  const float b = dot(v3(q.ocx, q.ocy, q.ocz), d);
+  const float disc = b * b - q.c;
+  if (disc >= 0.f) {
+    const float t = -b - sqrtf(disc);
+    if (t > 1e-4f && t < h.t) h = Hit{t, 1, i};
+  }
+}
+__device__ __forceinline__ Hit nearest_primary(const Scene& sc, V3 d) {
+  Hit h{1e30f, 0, -1};
+  const ConstTerms pre = (ConstTerms)sc.pre;
+  const ConstPlane pl = (ConstPlane)sc.pl;
+  constexpr int32_t kGroup = 8;
+  int32_t i = 0;
+  for (; i + kGroup <= sc.ns; i += kGroup) {
+    PrimaryTerms q[kGroup];
+#pragma unroll
+    for (int32_t k = 0; k < kGroup; ++k) q[k] = PrimaryTerms{pre[i + k].ocx, pre[i + k].ocy, pre[i + k].ocz, pre[i + k].c};
+#pragma unroll
+    for (int32_t k = 0; k < kGroup; ++k) sphere_test_pre(h, i + k, d, q[k]);
+  }
+  for (; i < sc.ns; ++i) sphere_test_pre(h, i, d, PrimaryTerms{pre[i].ocx, pre[i].ocy, pre[i].ocz, pre[i].c});
+  for (int32_t j = 0; j < sc.np; ++j) {
+    const auto& p = pl[j];
+    const float denom = dot(v3(p.n[0], p.n[1], p.n[2]), d);
+    if (fabsf(denom) > 1e-5f) {
+      const float t = -pre[sc.ns + j].ocx / denom;
+      if (t > 1e-4f && t < h.t) h = Hit{t, 2, j};
     }
   }
   return h;
@@ -135,7 +218,8 @@ __device__ __forceinline__ Ray primary(const Scene& sc, int32_t W, int32_t H, in
 // One bounce of ray r (k_trace body, rt_final.cpp:572-628).  Returns true
 // and updates r in place if the chain continues.
 __device__ __forceinline__ bool shade(const Scene& sc, int32_t max_depth, V3 light, Ray& r) {
-  const Hit h = nearest(sc, r.o, r.d);
+  // depth 0: a primary ray, whose origin is the camera's (primary())
+  const Hit h = sc.pre && r.depth == 0 ? nearest_primary(sc, r.d) : nearest(sc, r.o, r.d);
   if (h.kind == 0) {
     const float t = 0.5f * (r.d.y + 1.f);
     const V3 sky = v3(1.f, 1.f, 1.f) * (1.f - t) + v3(0.5f, 0.7f, 1.f) * t;

@@ -68,6 +68,22 @@ def test_mix_equals_alone(gcq, orc, world, lanes):
         _check_results(orc, srv, rows, comps, graphs, scene, csrs, full)
 
 
+@pytest.mark.parametrize("lanes", [1, "ms"], ids=["one_lane", "two_lanes_ms"])
+def test_sparse_arrivals_after_idle_parks(gcq, orc, world, lanes):
+    # Requests about 15 ms apart: between them every lane is idle and parks
+    # level after level; each arrival must still be admitted and exact.
+    from gcq_amd import serve
+    graphs, scene, csrs = world
+    nvs = [len(c[0]) - 1 for c in csrs]
+    rows = serve.poisson_trace(16, 66.0, nvs, FRAME, rt_fraction=0.25, seed=12, keep=1)
+    print(f"[serve sparse] {len(rows)} requests over {rows[-1]['arrival_ms']:.0f} ms, {lanes} lane(s)", flush=True)
+    with serve.Server(graphs, scene, FRAME, budget_bytes=1 << 30, keep_bytes=1 << 30, bfs_threads=16384,
+                      rt_threads=8192, frontier_log2=22, **LANES[lanes]) as srv:
+        comps = srv.run_trace(rows)
+        print("[serve sparse]", serve.latency_summary(comps), flush=True)
+        _check_results(orc, srv, rows, comps, graphs, scene, csrs)
+
+
 @pytest.mark.parametrize("lanes", [1, 2, "ms"], ids=["one_lane", "two_lanes", "two_lanes_ms"])
 @pytest.mark.parametrize("kind", [0, 1, 2, 3], ids=["gwfq", "glfq", "wfq", "sfq"])
 def test_burst_with_tight_limits(gcq, orc, world, kind, lanes):
