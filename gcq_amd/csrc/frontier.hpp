@@ -48,12 +48,12 @@ struct FGwf {
   // Head and Tail words pack (count << thr_bits | thread index); the index
   // field is the null index whenever no slow-path request is being
   // published, as at a level's start and end.
-  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::gwf::tail(r)) >> r->g.thr_bits; }
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::gwf::tail(r)) >> gcq::ld_const(&r->g).thr_bits; }
   __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
     using namespace gcq::gwf;
     NoProbe np;
     RingOps<> o(r, tid, &np);
-    const uint64_t T = o.cnt_of(gcq::wave_faa(head(r), (uint64_t)1 << r->g.thr_bits));
+    const uint64_t T = o.cnt_of(gcq::wave_faa(head(r), (uint64_t)1 << gcq::ld_const(&r->g).thr_bits));
     if (T >= end) return -1;
     return take(tid, T, end, peek(T), v);
   }
@@ -81,7 +81,7 @@ struct FGwf {
   // end: the level's end (level_end() at its start): the input ring's Tail,
   // which no one moves during the level, so the barrier need not read it.
   __device__ __forceinline__ void drained(uint64_t end) const {
-    const uint32_t B = r->g.thr_bits;
+    const uint32_t B = gcq::ld_const(&r->g).thr_bits;
     gcq::st_rlx(gcq::gwf::head(r), (end << B) | ((1ull << B) - 1ull));
   }
   __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
@@ -118,14 +118,14 @@ struct FLfq {
   }
   __device__ __forceinline__ uint64_t peek(uint64_t T) const {
     using namespace gcq::lfq;
-    const Geometry g = r->g;
+    const Geometry g = gcq::ld_const(&r->g);
     return gcq::ld_rlx(slots(r) + slot_of(T, g.slots_log2, g.remap_log2));
   }
   // Ticket T with its slot word E as first read: as pop_counted after its FAA.
   __device__ __forceinline__ int take(uint32_t, uint64_t T, uint64_t end, uint64_t E, uint32_t* v) const {
     using namespace gcq::lfq;
     if (T >= end) return -1;
-    const Geometry g = r->g;
+    const Geometry g = gcq::ld_const(&r->g);
     const uint32_t tcyc = cycle_of(T, g.slots_log2);
     uint64_t* const e = slots(r) + slot_of(T, g.slots_log2, g.remap_log2);
     // Written before the barrier; the wait only guards against a ticket an
@@ -165,9 +165,9 @@ struct FSfq {
   static constexpr bool kHeadFromDrain = true;
   gcq::sfq::Queue* q;
   __device__ __forceinline__ uint64_t* word(uint64_t t) const {
-    return &gcq::sfq::slots(q)[t & (q->g.qlen - 1)].id;
+    return &gcq::sfq::slots(q)[t & (gcq::ld_const(&q->g).qlen - 1)].id;
   }
-  __device__ __forceinline__ uint32_t lap_phase(uint64_t t) const { return (uint32_t)(2ull * (t >> q->g.qlen_log2)); }
+  __device__ __forceinline__ uint32_t lap_phase(uint64_t t) const { return (uint32_t)(2ull * (t >> gcq::ld_const(&q->g).qlen_log2)); }
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
   // Counted pops: the slot of a ticket below the level's end carries its
@@ -204,7 +204,7 @@ struct FSfq {
     const uint32_t empty = lap_phase(t);
     for (uint64_t k = 0; (uint32_t)(gcq::ld_rlx(word(t)) >> 32) != empty; ++k) {
       if (k > (1ull << 22)) return false;
-      gcq::sfq::slot_wait(k, q->g.backoff);
+      gcq::sfq::slot_wait(k, gcq::ld_const(&q->g).backoff);
     }
     gcq::st_rlx(word(t), ((uint64_t)(empty + 1u) << 32) | v);
     return true;
@@ -273,7 +273,7 @@ struct FYmc {  // values are v + 1: 0 is the queue's empty cell (wfqueue_hip_opt
     using namespace gcq::ymc;
     const uint64_t T = gcq::wave_faa(Hp(q), (uint64_t)1);
     if (T >= end) return -1;
-    if (T >= q->g.cells) return 0;
+    if (T >= gcq::ld_const(&q->g).cells) return 0;
     uint64_t* const c = &vals(q)[T];
     uint64_t x = gcq::ld_rlx(c);
     for (uint32_t w = 0; x == kBottom && w < gcq::kSlotWaitSpins; ++w) {

@@ -41,6 +41,30 @@ template <class T>
 __device__ __forceinline__ T ld_cached(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// Read of a queue's geometry header: written only by the queue's init
+// kernels, never while another kernel uses the queue, so it is read through
+// the constant address space.  The compiler may then load it with scalar
+// loads and keep it across a kernel's loop; a plain pointer made it reload
+// the header with a vector load and wait before every operation's first
+// atomic, since any store or atomic in the loop might have changed it.
+// Init code, which writes the header and reads it back in the same kernel,
+// reads it plainly.  Used by SFQ, WFQ and the BFS frontier adapters.  The
+// G-WFQ and G-LFQ rings keep plain reads: measured in the split 25/75
+// throughput run, constant reads there gained 7-16% at 16,384 and 32,768
+// threads but lost 15-17% at 65,536 with claims combined per workgroup
+// (3,230 -> 2,600 Mops/s for G-WFQ), and neither a sleep in their place nor
+// consumer backoff recovered it.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+  typedef const __attribute__((address_space(4))) uint32_t* W;
+  const W w = (W)p;
+  T out;
+  uint32_t* o = (uint32_t*)&out;
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; ++i) o[i] = w[i];  // unused words are dropped by the compiler
+  return out;
+}
 // Agent-scope acquire after a successful dequeue through the drop-in
 // headers' index and value APIs, for callers that read the dequeued index's
 // payload with plain loads (the ray tracing harness publishes ray records

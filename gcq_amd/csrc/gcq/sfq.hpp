@@ -48,9 +48,9 @@ static_assert(sizeof(Queue) == kHdrBytes, "header size");
 __host__ __device__ constexpr uint64_t queue_bytes(uint64_t qlen) { return kHdrBytes + 16ull * qlen; }
 
 __device__ __forceinline__ Slot* slots(Queue* q) { return (Slot*)((char*)q + kHdrBytes); }
-__device__ __forceinline__ uint64_t* tail(Queue* q) { return (uint64_t*)((char*)q + q->g.off_tail); }
-__device__ __forceinline__ uint64_t* head(Queue* q) { return (uint64_t*)((char*)q + q->g.off_head); }
-__device__ __forceinline__ uint32_t* closed(Queue* q) { return (uint32_t*)((char*)q + q->g.off_closed); }
+__device__ __forceinline__ uint64_t* tail(Queue* q) { return (uint64_t*)((char*)q + ld_const(&q->g).off_tail); }
+__device__ __forceinline__ uint64_t* head(Queue* q) { return (uint64_t*)((char*)q + ld_const(&q->g).off_head); }
+__device__ __forceinline__ uint32_t* closed(Queue* q) { return (uint32_t*)((char*)q + ld_const(&q->g).off_closed); }
 
 struct NoProbe {
   __device__ __forceinline__ void step() {}
@@ -102,7 +102,7 @@ __device__ __forceinline__ void slot_wait(uint64_t spins, uint32_t iters) {
 template <class Probe = NoProbe>
 __device__ inline int enqueue_blocking(Queue* q, uint64_t item, uint64_t spin_limit = 0, Probe* pr = nullptr) {
   if (is_closed(q)) return kClosed;
-  const Geometry g = q->g;
+  const Geometry g = ld_const(&q->g);
   const uint64_t t = wave_faa(tail(q), (uint64_t)1);
   if (pr) { pr->step(); pr->after_ticket(); }
   Slot* s = &slots(q)[t & (g.qlen - 1)];
@@ -122,7 +122,7 @@ __device__ inline int enqueue_blocking(Queue* q, uint64_t item, uint64_t spin_li
 template <class Probe = NoProbe>
 __device__ inline int dequeue_blocking(Queue* q, uint64_t* out, uint64_t spin_limit = 0, Probe* pr = nullptr) {
   if (is_closed(q)) return kClosed;
-  const Geometry g = q->g;
+  const Geometry g = ld_const(&q->g);
   const uint64_t t = wave_faa(head(q), (uint64_t)1);
   if (pr) { pr->step(); pr->after_ticket(); }
   Slot* s = &slots(q)[t & (g.qlen - 1)];
@@ -158,7 +158,7 @@ __device__ __forceinline__ uint8_t* busy_backoff() {
 }
 
 __device__ __forceinline__ uint64_t nb_claim(Queue* q, uint64_t* ctr, uint64_t phase_add) {
-  const Geometry g = q->g;
+  const Geometry g = ld_const(&q->g);
   const uint64_t m = active_mask();
   const uint32_t leader = first_lane(m);
   uint64_t t0 = 0;
@@ -202,7 +202,7 @@ __device__ __forceinline__ uint64_t nb_claim(Queue* q, uint64_t* ctr, uint64_t p
 // progress between the read and the CAS, as in the per-wave claim.
 constexpr uint32_t kGroupProbeRounds = 4;
 __device__ inline uint64_t nb_claim_group(Queue* q, uint64_t* ctr, uint64_t phase_add, Combine* c) {
-  const Geometry g = q->g;
+  const Geometry g = ld_const(&q->g);
   const uint64_t m = active_mask();
   const uint32_t leader = first_lane(m);
   const uint32_t n = (uint32_t)__builtin_popcountll(m);
@@ -300,10 +300,10 @@ __device__ inline int enqueue_nb(Queue* q, uint64_t item, Probe* pr = nullptr, C
   const uint64_t t = cg ? nb_claim_group(q, tail(q), 0, cg) : nb_claim(q, tail(q), 0);
   if (pr) pr->step();
   if (t == ~0ull) return kBusy;
-  Slot* s = &slots(q)[t & (q->g.qlen - 1)];
+  Slot* s = &slots(q)[t & (ld_const(&q->g).qlen - 1)];
   st_rlx(&s->item, item);
   vm_wait();
-  st_rlx(&s->id, 2ull * (t >> q->g.qlen_log2) + 1ull);
+  st_rlx(&s->id, 2ull * (t >> ld_const(&q->g).qlen_log2) + 1ull);
   return kSuccess;
 }
 
@@ -315,10 +315,10 @@ __device__ inline int dequeue_nb(Queue* q, uint64_t* out, Probe* pr = nullptr, C
   const uint64_t t = cg ? nb_claim_group(q, head(q), 1, cg) : nb_claim(q, head(q), 1);
   if (pr) pr->step();
   if (t == ~0ull) return kBusy;
-  Slot* s = &slots(q)[t & (q->g.qlen - 1)];
+  Slot* s = &slots(q)[t & (ld_const(&q->g).qlen - 1)];
   *out = ld_rlx(&s->item);
   vm_wait();
-  st_rlx(&s->id, 2ull * (t >> q->g.qlen_log2) + 2ull);
+  st_rlx(&s->id, 2ull * (t >> ld_const(&q->g).qlen_log2) + 2ull);
   return kSuccess;
 }
 

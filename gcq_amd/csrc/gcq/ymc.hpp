@@ -131,17 +131,17 @@ __host__ __device__ constexpr uint64_t queue_bytes(uint64_t cells) { return kHdr
 // The values: g.cells u64 words after the header (or at ext_cells), then
 // the reference pairs, g.cells more u64 words.
 __device__ __forceinline__ uint64_t* vals(Queue* q) {
-  const uint64_t ext = q->g.ext_cells;
+  const uint64_t ext = ld_const(&q->g).ext_cells;
   return ext ? (uint64_t*)ext : (uint64_t*)((char*)q + kHdrBytes);
 }
-__device__ __forceinline__ Refs* refs(Queue* q) { return (Refs*)(vals(q) + q->g.cells); }
+__device__ __forceinline__ Refs* refs(Queue* q) { return (Refs*)(vals(q) + ld_const(&q->g).cells); }
 __device__ __forceinline__ Cell cell_at(Queue* q, uint64_t k) {
   Refs* r = refs(q) + k;
   return Cell{vals(q) + k, &r->enq, &r->deq};
 }
-__device__ __forceinline__ uint64_t* Tp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_t); }
-__device__ __forceinline__ uint64_t* Hp(Queue* q) { return (uint64_t*)((char*)q + q->g.off_h); }
-__device__ __forceinline__ Window* win(Queue* q) { return (Window*)((char*)q + q->g.off_win); }
+__device__ __forceinline__ uint64_t* Tp(Queue* q) { return (uint64_t*)((char*)q + ld_const(&q->g).off_t); }
+__device__ __forceinline__ uint64_t* Hp(Queue* q) { return (uint64_t*)((char*)q + ld_const(&q->g).off_h); }
+__device__ __forceinline__ Window* win(Queue* q) { return (Window*)((char*)q + ld_const(&q->g).off_win); }
 // Ring mode's clearing job: the dead cells [0, end) (logical ids) are
 // cleared in chunks claimed at `next`; `cleared` counts the cleared cells.
 // A new job (a higher end) is published only once cleared == end.
@@ -149,11 +149,11 @@ struct ClearJob {
   uint64_t end, next, cleared, pad[13];
 };
 static_assert(sizeof(ClearJob) == kLine, "one line");
-__device__ __forceinline__ ClearJob* job(Queue* q) { return (ClearJob*)((char*)q + q->g.off_bound); }
-__device__ __forceinline__ uint32_t* lockp(Queue* q) { return (uint32_t*)((char*)q + q->g.off_lock); }
+__device__ __forceinline__ ClearJob* job(Queue* q) { return (ClearJob*)((char*)q + ld_const(&q->g).off_bound); }
+__device__ __forceinline__ uint32_t* lockp(Queue* q) { return (uint32_t*)((char*)q + ld_const(&q->g).off_lock); }
 // Ring mode: one published floor per wave (thread id / 64), after the handles.
 __host__ __device__ constexpr uint64_t wave_slots(uint32_t nthreads) { return (nthreads + 63u) / 64u; }
-__device__ __forceinline__ uint64_t* wave_lo(Queue* q, Handle* hs) { return (uint64_t*)(hs + q->g.nthreads); }
+__device__ __forceinline__ uint64_t* wave_lo(Queue* q, Handle* hs) { return (uint64_t*)(hs + ld_const(&q->g).nthreads); }
 // Per wave of a workgroup, in LDS: whether the wave has published its floor
 // in this kernel, and the value it published (0 after a helper lowered it).
 // avail and next_try are cached here too and read again every
@@ -281,9 +281,9 @@ struct Ops {
   // first operation in a kernel publishes its floor and waits for the store
   // to complete before it takes any ticket.
   __device__ __forceinline__ void prologue() {
-    ring = q->g.ring;
+    ring = ld_const(&q->g).ring;
     if (!ring) return;
-    seg_log2 = q->g.seg_log2;
+    seg_log2 = ld_const(&q->g).seg_log2;
     const uint32_t L = first_lane(active_mask());
     uint64_t a = 0, nt = 0, cd = 0;
     if (lane_id() == L) {
@@ -415,7 +415,7 @@ struct Ops {
     if (!bcast32(busy, L)) {
       const uint64_t* lo = wave_lo(q, hs);
       uint64_t mn = kIdle;
-      for (uint64_t k = r; k < wave_slots(q->g.nthreads); k += n) {
+      for (uint64_t k = r; k < wave_slots(ld_const(&q->g).nthreads); k += n) {
         const uint64_t v = ld_rlx(&lo[k]);
         mn = v < mn ? v : mn;
       }
@@ -750,7 +750,7 @@ struct Ops {
   // costs nothing until the wave's last T ticket comes near the end.
   __device__ bool room_to_enqueue() {
     if (!ring) return true;
-    const uint64_t margin = (q->g.nthreads >> seg_log2) + 2;
+    const uint64_t margin = (ld_const(&q->g).nthreads >> seg_log2) + 2;
     const uint32_t L = first_lane(active_mask());
     const uint64_t a0 = avail;  // wave-uniform
     uint64_t a = a0;
@@ -785,7 +785,7 @@ struct Ops {
 
   __device__ bool enqueue_body(uint64_t v, Combine* ct) {  // :527-536
     if (!room_to_enqueue()) return false;  // ring mode: full
-    for (int p = (int)q->g.patience; p >= 0; --p) {
+    for (int p = (int)ld_const(&q->g).patience; p >= 0; --p) {
       uint64_t cid = 0;
       if (enq_fast(v, &cid, ct)) return true;
       if (cid == kWithdrawn) return false;  // pool exhausted
@@ -811,7 +811,7 @@ struct Ops {
     // every enqueue cell is already owned by a dequeuer (EMPTY at the T
     // read).  Idle consumers would otherwise burn a cell per attempt.
     uint64_t v = kRetry, cid = 0;
-    for (int p = (int)q->g.patience; p >= 0; --p) {
+    for (int p = (int)ld_const(&q->g).patience; p >= 0; --p) {
       if (!claim_h(&cid)) return kEmpty;
       v = deq_fast(cid);
       if (v != kRetry) break;
@@ -831,13 +831,13 @@ template <class Probe = NoProbe>
 __device__ __forceinline__ bool enqueue(Queue* q, Handle* hs, uint32_t tid, uint64_t v, Probe* pr = nullptr,
                                         Combine* ct = nullptr) {
   NoProbe np;
-  Ops<Probe> o{q, hs, &hs[tid], tid, q->g.cells, pr ? pr : (Probe*)&np};
+  Ops<Probe> o{q, hs, &hs[tid], tid, ld_const(&q->g).cells, pr ? pr : (Probe*)&np};
   return o.enqueue(v, ct);
 }
 template <class Probe = NoProbe>
 __device__ __forceinline__ uint64_t dequeue(Queue* q, Handle* hs, uint32_t tid, Probe* pr = nullptr) {
   NoProbe np;
-  Ops<Probe> o{q, hs, &hs[tid], tid, q->g.cells, pr ? pr : (Probe*)&np};
+  Ops<Probe> o{q, hs, &hs[tid], tid, ld_const(&q->g).cells, pr ? pr : (Probe*)&np};
   return o.dequeue();
 }
 // Ring mode: every wave of a kernel that operates on the queue calls
@@ -846,11 +846,11 @@ __device__ __forceinline__ uint64_t dequeue(Queue* q, Handle* hs, uint32_t tid, 
 // wave's floor unpublished in LDS; end() withdraws a published floor, so a
 // wave that stops using the queue does not hold reclamation back.
 __device__ __forceinline__ void begin(Queue* q) {
-  if (!q->g.ring) return;
+  if (!ld_const(&q->g).ring) return;
   if (lane_id() == first_lane(active_mask())) wave_lds()->pub[wave_in_block()] = 0;
 }
 __device__ __forceinline__ void end(Queue* q, Handle* hs, uint32_t tid) {
-  if (!q->g.ring) return;
+  if (!ld_const(&q->g).ring) return;
   if (lane_id() == first_lane(active_mask())) {
     WaveLds* s = wave_lds();
     const uint32_t wv = wave_in_block();
@@ -872,7 +872,7 @@ __device__ __forceinline__ uint64_t high_water(Queue* q) {
 // then one thread calls rewind() once all of them are done.  Shared words
 // are cleared with device-scope stores so waves on other XCDs see them.
 __device__ inline void recycle_cells(Queue* q, uint64_t gid, uint64_t stride) {
-  const uint64_t hw = min(high_water(q), q->g.cells);
+  const uint64_t hw = min(high_water(q), ld_const(&q->g).cells);
   uint64_t* const vw = vals(q);
   uint64_t* const rw = (uint64_t*)refs(q);
   for (uint64_t i = gid; i < hw; i += stride) {
@@ -881,7 +881,7 @@ __device__ inline void recycle_cells(Queue* q, uint64_t gid, uint64_t stride) {
   }
 }
 __device__ inline void reset_handle(Queue* q, Handle* hs, uint32_t tid) {
-  const uint32_t n = q->g.nthreads;
+  const uint32_t n = ld_const(&q->g).nthreads;
   if (tid >= n) return;
   Handle* h = &hs[tid];
   st_rlx(&h->enq_val, 0ull);
