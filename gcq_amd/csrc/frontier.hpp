@@ -108,11 +108,11 @@ struct FLfq {
   __device__ __forceinline__ void rewind() const {}
   // End of the level's tickets: the input ring's Tail, which no one moves
   // during the level.
-  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::lfq::tail(r)); }
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::lfq::tail_p<true>(r)); }
   // 1 = *v popped, 0 = this ticket holds no item (skip it), -1 = the level
   // is drained.
   __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
-    const uint64_t T = gcq::wave_faa(gcq::lfq::head(r), (uint64_t)1);
+    const uint64_t T = gcq::wave_faa(gcq::lfq::head_p<true>(r), (uint64_t)1);
     if (T >= end) return -1;
     return take(tid, T, end, peek(T), v);
   }
@@ -139,13 +139,13 @@ struct FLfq {
     *v = e_idx(E);
     return 1;
   }
-  __device__ __forceinline__ void drained(uint64_t end) const { gcq::st_rlx(gcq::lfq::head(r), end); }
+  __device__ __forceinline__ void drained(uint64_t end) const { gcq::st_rlx(gcq::lfq::head_p<true>(r), end); }
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
-    gcq::lfq::enqueue(r, v);
+    gcq::lfq::enqueue<gcq::lfq::NoProbe, true>(r, v);
     return true;
   }
   __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
-    const uint32_t x = gcq::lfq::dequeue(r, (gcq::lfq::NoProbe*)nullptr, true);
+    const uint32_t x = gcq::lfq::dequeue<gcq::lfq::NoProbe, true>(r, nullptr, true);
     if (x == gcq::lfq::kBot) return false;
     *v = x;
     return true;

@@ -50,6 +50,31 @@ __device__ __forceinline__ uint64_t* slots(Ring* r) { return (uint64_t*)((char*)
 __device__ __forceinline__ uint64_t* tail(Ring* r) { return (uint64_t*)((char*)r + r->g.off_tail); }
 __device__ __forceinline__ uint64_t* head(Ring* r) { return (uint64_t*)((char*)r + r->g.off_head); }
 __device__ __forceinline__ int32_t* thr(Ring* r) { return (int32_t*)((char*)r + r->g.off_thr); }
+// The same with the geometry read through the constant address space
+// (ld_const) when kC: the BFS frontiers and the ray tracer's tile queues use
+// it; the throughput, history and FIFO harnesses keep plain reads (measured:
+// constant reads cost the split 25/75 run 15% at 65,536 threads with
+// combined claims, while they shorten the frontier and tile queues' chains).
+template <bool kC>
+__device__ __forceinline__ Geometry geom(Ring* r) {
+  if constexpr (kC) return ld_const(&r->g);
+  else return r->g;
+}
+template <bool kC>
+__device__ __forceinline__ uint64_t* tail_p(Ring* r) {
+  if constexpr (kC) return (uint64_t*)((char*)r + ld_const(&r->g).off_tail);
+  else return tail(r);
+}
+template <bool kC>
+__device__ __forceinline__ uint64_t* head_p(Ring* r) {
+  if constexpr (kC) return (uint64_t*)((char*)r + ld_const(&r->g).off_head);
+  else return head(r);
+}
+template <bool kC>
+__device__ __forceinline__ int32_t* thr_p(Ring* r) {
+  if constexpr (kC) return (int32_t*)((char*)r + ld_const(&r->g).off_thr);
+  else return thr(r);
+}
 
 __host__ __device__ __forceinline__ uint32_t e_cyc(uint64_t e) { return (uint32_t)(e >> kCycShift) & 0xFFu; }
 __host__ __device__ __forceinline__ bool e_safe(uint64_t e) { return (e >> kSafeShift) & 1u; }
@@ -113,7 +138,8 @@ __device__ inline void init_slice(Ring* r, uint32_t n, bool full, uint32_t remap
 // times (glf_queue.hpp:447-455); every failed CAS is another request to the
 // hottest line of the ring.  Tail is a plain counter here, so one fetch_max
 // has the same effect.
-__device__ __forceinline__ void catchup(Ring* r, uint64_t h) { max_rlx(tail(r), h); }
+template <bool kC = false>
+__device__ __forceinline__ void catchup(Ring* r, uint64_t h) { max_rlx(tail_p<kC>(r), h); }
 
 // Head ticket bounded by the items present (consumer side).  The wave
 // leader reads Head, then Tail, and takes k = min(active lanes, Tail - Head)
@@ -125,6 +151,7 @@ __device__ __forceinline__ void catchup(Ring* r, uint64_t h) { max_rlx(tail(r), 
 // (Tail <= Head at the leader's Tail read: EMPTY for the whole wave,
 // linearised at that read).
 constexpr uint64_t kNoTicket = ~0ull, kEmptyTicket = ~1ull;
+template <bool kC = false>
 __device__ __forceinline__ uint64_t claim_head(Ring* r) {
   const uint64_t m = active_mask();
   const uint32_t leader = first_lane(m);
@@ -132,12 +159,12 @@ __device__ __forceinline__ uint64_t claim_head(Ring* r) {
   uint64_t base = 0;
   uint32_t k = 0;
   if (lane_id() == leader) {
-    const uint64_t h = ld_rlx(head(r));
+    const uint64_t h = ld_rlx(head_p<kC>(r));
     vm_wait();  // Head is read before Tail
-    const uint64_t t = ld_rlx(tail(r));
+    const uint64_t t = ld_rlx(tail_p<kC>(r));
     const uint64_t avail = t > h ? t - h : 0;
     k = avail < n ? (uint32_t)avail : n;
-    if (k) base = faa_rlx(head(r), (uint64_t)k);
+    if (k) base = faa_rlx(head_p<kC>(r), (uint64_t)k);
   }
   k = bcast32(k, leader);
   if (k == 0) return kEmptyTicket;
@@ -148,6 +175,7 @@ __device__ __forceinline__ uint64_t claim_head(Ring* r) {
 
 // claim_head() with the Head/Tail reads and the FAA shared by the waves of
 // the workgroup that claim together (combine_claim).
+template <bool kC = false>
 __device__ __forceinline__ uint64_t claim_head_combined(Ring* r, Combine* cb) {
   const uint64_t m = active_mask();
   const uint32_t leader = first_lane(m);
@@ -156,12 +184,12 @@ __device__ __forceinline__ uint64_t claim_head_combined(Ring* r, Combine* cb) {
   uint32_t k = 0, off = 0;
   if (lane_id() == leader) {
     off = combine_claim(cb, n, &k, &base, [&](uint32_t lanes, uint32_t* kk, uint64_t* bb) {
-      const uint64_t h = ld_rlx(head(r));
+      const uint64_t h = ld_rlx(head_p<kC>(r));
       vm_wait();  // Head is read before Tail
-      const uint64_t t = ld_rlx(tail(r));
+      const uint64_t t = ld_rlx(tail_p<kC>(r));
       const uint64_t avail = t > h ? t - h : 0;
       *kk = avail < lanes ? (uint32_t)avail : lanes;
-      *bb = *kk ? faa_rlx(head(r), (uint64_t)*kk) : 0;
+      *bb = *kk ? faa_rlx(head_p<kC>(r), (uint64_t)*kk) : 0;
     });
   }
   k = bcast32(k, leader);
@@ -182,13 +210,13 @@ struct NoProbe {
 // scq_reset: reset the threshold in SCQ's load-then-store form (the
 // free-index ring; reset_threshold).  ct: draw the Tail ticket together with
 // the workgroup's other enqueuing waves (group_faa).
-template <class Probe = NoProbe>
+template <class Probe = NoProbe, bool kC = false>
 __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool scq_reset = false,
                                Combine* ct = nullptr) {
-  const Geometry g = r->g;
+  const Geometry g = geom<kC>(r);
   uint64_t* const e = slots(r);
   for (;;) {
-    const uint64_t T = ct ? group_faa(ct, tail(r), (uint64_t)1) : wave_faa(tail(r), (uint64_t)1);
+    const uint64_t T = ct ? group_faa(ct, tail_p<kC>(r), (uint64_t)1) : wave_faa(tail_p<kC>(r), (uint64_t)1);
     // Every earlier memory operation of this wave completes before the slot
     // is touched; this is what orders a payload store before the index is
     // published (and a payload load before the index is recycled).
@@ -204,7 +232,7 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool 
     bool ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));  // E refreshed on failure
     while (!ok) {
       if (pr) pr->step();
-      if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || ld_rlx(head(r)) <= T))) break;
+      if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || ld_rlx(head_p<kC>(r)) <= T))) break;
       ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));
     }
     // Threshold back to 3n - 1 after the insertions (glf_queue.hpp:493-495):
@@ -213,7 +241,7 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool 
     // when it already holds 3n - 1 is harmless, and a reset that lands after
     // a concurrent decrement is the same as a late reset.
     const uint64_t okm = __ballot(ok);
-    if (okm && lane_id() == first_lane(okm)) reset_threshold(thr(r), g.thresh_max, scq_reset);
+    if (okm && lane_id() == first_lane(okm)) reset_threshold(thr_p<kC>(r), g.thresh_max, scq_reset);
     if (ok) return;
   }
 }
@@ -226,15 +254,15 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool 
 // not mark free-ring
 // slots ahead of the consumers returning indices.  The per-wave hint only
 // picks the ticket path; its initial LDS contents do not matter.
-template <class Probe = NoProbe>
+template <class Probe = NoProbe, bool kC = false>
 __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check = true, Combine* cb = nullptr) {
-  const Geometry g = r->g;
+  const Geometry g = geom<kC>(r);
   uint64_t* const e = slots(r);
   __shared__ uint8_t s_low[16];  // per wave: the free ring was found empty lately
   const uint32_t wv = ((threadIdx.x + threadIdx.y * blockDim.x) / kWave) & 15u;
   bool bounded = idle_check;
   if (!idle_check) {
-    if (ld_rlx(thr(r)) < 0) {  // consumer side: the claim tests emptiness
+    if (ld_rlx(thr_p<kC>(r)) < 0) {  // consumer side: the claim tests emptiness
       s_low[wv] = low_next(s_low[wv], true);
       return kBot;
     }
@@ -248,10 +276,10 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
       if (cb) {
         uint8_t* const recent = combine_recent();
         const uint32_t rc = *recent & 7u;
-        H = rc ? claim_head_combined(r, cb) : claim_head(r);
+        H = rc ? claim_head_combined<kC>(r, cb) : claim_head<kC>(r);
         *recent = (uint8_t)(H == kEmptyTicket ? kCombineAfterEmpty : (rc ? rc - 1u : 0u));
       } else {
-        H = claim_head(r);
+        H = claim_head<kC>(r);
       }
       if (H == kEmptyTicket) {
         if (idle_check) idle_note(g.poll_backoff, true);
@@ -263,7 +291,7 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
         continue;
       }
     } else {
-      H = wave_faa(head(r), (uint64_t)1);
+      H = wave_faa(head_p<kC>(r), (uint64_t)1);
     }
     if (pr) { pr->step(); pr->after_ticket(); }
     const uint32_t hcyc = cycle_of(H, g.slots_log2);
@@ -291,14 +319,14 @@ __device__ inline uint32_t dequeue(Ring* r, Probe* pr = nullptr, bool idle_check
       }
       break;
     }
-    const uint64_t T = ld_rlx(tail(r));
+    const uint64_t T = ld_rlx(tail_p<kC>(r));
     if (T <= H + 1) {
-      catchup(r, H + 1);
-      faa_rlx(thr(r), -1);
+      catchup<kC>(r, H + 1);
+      faa_rlx(thr_p<kC>(r), -1);
       if (!idle_check) s_low[wv] = low_next(s_low[wv], true);
       return kBot;
     }
-    if (faa_rlx(thr(r), -1) <= 0) {
+    if (faa_rlx(thr_p<kC>(r), -1) <= 0) {
       if (!idle_check) s_low[wv] = low_next(s_low[wv], true);
       return kBot;
     }
