@@ -310,14 +310,21 @@ int rtk::scene_upload(gcq_rt_scene* s) {
     GCQ_HIP(hipMemcpy(s->d_sph, s->sph.data(), sizeof(gcq_rt_sphere) * s->sph.size(), hipMemcpyHostToDevice));
   if (!s->pl.empty())
     GCQ_HIP(hipMemcpy(s->d_pl, s->pl.data(), sizeof(gcq_rt_plane) * s->pl.size(), hipMemcpyHostToDevice));
+  // The camera's primary terms; the renderers use them only once they are
+  // complete (d_pre stays null otherwise, and every ray takes the full test).
   const int32_t nobj = (int32_t)(s->sph.size() + s->pl.size());
-  GCQ_HIP(hipMalloc(&s->d_pre, sizeof(rtk::PrimaryTerms) * std::max<int32_t>(1, nobj)));
   if (nobj) {
+    rtk::PrimaryTerms* pre = nullptr;
+    GCQ_HIP(hipMalloc(&pre, sizeof(rtk::PrimaryTerms) * (size_t)nobj));
     const float* o = s->desc.cam_o;
     hipLaunchKernelGGL(rtk::k_primary_terms, dim3((nobj + 255) / 256), dim3(256), 0, s->stream, s->d_sph,
-                       (int32_t)s->sph.size(), s->d_pl, (int32_t)s->pl.size(), rtk::v3(o[0], o[1], o[2]), s->d_pre);
-    GCQ_HIP(hipGetLastError());
-    GCQ_HIP(hipStreamSynchronize(s->stream));
+                       (int32_t)s->sph.size(), s->d_pl, (int32_t)s->pl.size(), rtk::v3(o[0], o[1], o[2]), pre);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
+      (void)hipFree(pre);
+      gcqh::set_error("primary-ray terms kernel failed");
+      return GCQ_EHIP;
+    }
+    s->d_pre = pre;
   }
   return GCQ_OK;
 }
