@@ -583,13 +583,19 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
       gcqh::set_error("persistent BFS needs all %u blocks resident; the device holds %d", grid, per_cu * cus);
       rc = GCQ_EINVAL;
     } else {
+      // One push per vertex (its claim is won once), into rings of at least
+      // V slots reset before the run: the SFQ frontier skips its slot checks.
+      auto fqp = fq;
+      if constexpr (std::is_same_v<F, gcqf::FSfq>)
+        if (c.fs.capacity >= (uint64_t)g->host.nv)
+          for (auto& x : fqp.q) x.single_lap = 1u;
       GCQ_HIP(hipEventRecord(e0, g->stream));
       if (bitmap)
-        hipLaunchKernelGGL((bfsk::k_persistent<F, true>), dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off,
+        hipLaunchKernelGGL((bfsk::k_persistent<F, true>), dim3(grid), dim3(cfg->block), 0, g->stream, fqp, g->d_off,
                            g->d_idx, c.level, ctl, c.vis, hub);
       else
-        hipLaunchKernelGGL((bfsk::k_persistent<F, false>), dim3(grid), dim3(cfg->block), 0, g->stream, fq, g->d_off,
-                           g->d_idx, c.level, ctl, (uint32_t*)nullptr, hub);
+        hipLaunchKernelGGL((bfsk::k_persistent<F, false>), dim3(grid), dim3(cfg->block), 0, g->stream, fqp,
+                           g->d_off, g->d_idx, c.level, ctl, (uint32_t*)nullptr, hub);
       GCQ_HIP(hipEventRecord(e1, g->stream));
     }
   } else {

@@ -164,6 +164,11 @@ struct FSfq {
   static constexpr bool kCounted = true;
   static constexpr bool kHeadFromDrain = true;
   gcq::sfq::Queue* q;
+  // Set by a caller that pushes at most the ring's length into it between
+  // resets (one push per vertex into a ring of at least V slots): every
+  // ticket is then on lap 0, whose slots init left empty, and a push stores
+  // without first reading its slot's phase.
+  uint32_t single_lap = 0;
   __device__ __forceinline__ uint64_t* word(uint64_t t) const {
     return &gcq::sfq::slots(q)[t & (gcq::ld_const(&q->g).qlen - 1)].id;
   }
@@ -202,7 +207,7 @@ struct FSfq {
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
     const uint64_t t = gcq::wave_faa(gcq::sfq::tail(q), (uint64_t)1);
     const uint32_t empty = lap_phase(t);
-    for (uint64_t k = 0; (uint32_t)(gcq::ld_rlx(word(t)) >> 32) != empty; ++k) {
+    for (uint64_t k = 0; !single_lap && (uint32_t)(gcq::ld_rlx(word(t)) >> 32) != empty; ++k) {
       if (k > (1ull << 22)) return false;
       gcq::sfq::slot_wait(k, gcq::ld_const(&q->g).backoff);
     }
@@ -677,7 +682,7 @@ struct FrontierSet {
     for (int i = 0; i < 3; ++i) {
       if constexpr (std::is_same_v<F, FGwf>) f.q[i] = FGwf{(gcq::gwf::Ring*)q[i % 2]};
       else if constexpr (std::is_same_v<F, FLfq>) f.q[i] = FLfq{(gcq::lfq::Ring*)q[i % 2]};
-      else if constexpr (std::is_same_v<F, FSfq>) f.q[i] = FSfq{(gcq::sfq::Queue*)q[i % 2]};
+      else if constexpr (std::is_same_v<F, FSfq>) f.q[i] = FSfq{(gcq::sfq::Queue*)q[i % 2], 0u};
       else f.q[i] = FYmc{(gcq::ymc::Queue*)q[i], (gcq::ymc::Handle*)h[i]};
     }
     return f;
