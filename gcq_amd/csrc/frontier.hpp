@@ -166,8 +166,9 @@ struct FSfq {
   gcq::sfq::Queue* q;
   // Set by a caller that pushes at most the ring's length into it between
   // resets (one push per vertex into a ring of at least V slots): every
-  // ticket is then on lap 0, whose slots init left empty, and a push stores
-  // without first reading its slot's phase.
+  // ticket is then on lap 0, whose slots init left empty, so a push stores
+  // without first reading its slot's phase and a counted pop does not hand
+  // its slot on to a next lap.
   uint32_t single_lap = 0;
   __device__ __forceinline__ uint64_t* word(uint64_t t) const {
     return &gcq::sfq::slots(q)[t & (gcq::ld_const(&q->g).qlen - 1)].id;
@@ -196,7 +197,7 @@ struct FSfq {
       w = gcq::ld_rlx(word(T));
     }
     if ((uint32_t)(w >> 32) != full) return 0;
-    gcq::st_rlx(word(T), (uint64_t)(full + 1u) << 32);  // empty for the next lap
+    if (!single_lap) gcq::st_rlx(word(T), (uint64_t)(full + 1u) << 32);  // empty for the next lap (none: single lap)
     *v = (uint32_t)w;
     return 1;
   }
