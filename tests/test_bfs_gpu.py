@@ -74,6 +74,29 @@ def test_special_graphs(gcq, orc, kind):
 
 
 @pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
+def test_frontier_ring_of_exactly_v_slots(gcq, orc, kind):
+    # V = 4,096 with 1,024 threads: the frontier rings hold exactly V slots.
+    # The star pushes V - 1 vertices into one ring in one level, the path
+    # fills both rings over V levels, the complete binary tree in 12 levels;
+    # each graph runs twice, so the second run reuses the reset rings.  The
+    # SFQ frontier's persistent and bitmap modes rely on every ticket staying
+    # on the ring's first lap here.
+    from gcq_amd import bfs
+    n = 4096
+    a = np.arange(n - 1)
+    cases = {
+        "star": (_csr(n, np.zeros(n - 1, int), np.arange(1, n)), 0),
+        "path": (_csr(n, a, a + 1), 0),
+        "tree": (_csr(n, (np.arange(1, n) - 1) // 2, np.arange(1, n)), 0),
+    }
+    for name, ((off, idx), src) in cases.items():
+        print(f"[bfs V-slot rings] {name}", flush=True)
+        with bfs.Graph.from_csr(off, idx) as g:
+            for mode in (1, 3, 1):
+                _run_and_check(orc, g, src, kind, 1024, mode, off, idx)
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=[NAMES[k] for k in KINDS])
 def test_rmat_hubs(gcq, orc, kind):
     # Scale 15 R-MAT: frontiers well above the reference's 32,768-slot
     # rings and hub vertices of degree in the thousands.
