@@ -44,22 +44,23 @@ struct FGwf {
   static constexpr bool kHeadFromDrain = true;
   gcq::gwf::Ring* r;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
+  __device__ __forceinline__ uint64_t* o_tail() const { return (uint64_t*)((char*)r + gcq::ld_const(&r->g).off_tail); }
   __device__ __forceinline__ void rewind() const {}
   // Head and Tail words pack (count << thr_bits | thread index); the index
   // field is the null index whenever no slow-path request is being
   // published, as at a level's start and end.
-  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(gcq::gwf::tail(r)) >> gcq::ld_const(&r->g).thr_bits; }
+  __device__ __forceinline__ uint64_t level_end() const { return gcq::ld_rlx(o_tail()) >> gcq::ld_const(&r->g).thr_bits; }
   __device__ __forceinline__ int pop_counted(uint32_t tid, uint64_t end, uint32_t* v) const {
     using namespace gcq::gwf;
     NoProbe np;
-    RingOps<> o(r, tid, &np);
-    const uint64_t T = o.cnt_of(gcq::wave_faa(head(r), (uint64_t)1 << gcq::ld_const(&r->g).thr_bits));
+    RingOps<NoProbe, true> o(r, tid, &np);
+    const uint64_t T = o.cnt_of(gcq::wave_faa(o.head_(r), (uint64_t)1 << o.g.thr_bits));
     if (T >= end) return -1;
     return take(tid, T, end, peek(T), v);
   }
   __device__ __forceinline__ uint64_t peek(uint64_t T) const {
     gcq::gwf::NoProbe np;
-    gcq::gwf::RingOps<> o(r, 0, &np);
+    gcq::gwf::RingOps<gcq::gwf::NoProbe, true> o(r, 0, &np);
     return gcq::ld_rlx(&gcq::gwf::slots(r)[o.slot(T)]);
   }
   // Ticket T with its slot word E as first read: as pop_counted after its FAA.
@@ -67,7 +68,7 @@ struct FGwf {
     using namespace gcq::gwf;
     if (T >= end) return -1;
     NoProbe np;
-    RingOps<> o(r, tid, &np);
+    RingOps<NoProbe, true> o(r, tid, &np);
     const uint32_t j = o.slot(T), tcyc = o.cyc(T);
     for (uint32_t w = 0; (e_cyc(E) != tcyc || e_idx(E) >= kBot) && w < gcq::kSlotWaitSpins; ++w) {
       __builtin_amdgcn_s_sleep(1);
@@ -82,17 +83,17 @@ struct FGwf {
   // which no one moves during the level, so the barrier need not read it.
   __device__ __forceinline__ void drained(uint64_t end) const {
     const uint32_t B = gcq::ld_const(&r->g).thr_bits;
-    gcq::st_rlx(gcq::gwf::head(r), (end << B) | ((1ull << B) - 1ull));
+    gcq::st_rlx((uint64_t*)((char*)r + gcq::ld_const(&r->g).off_head), (end << B) | ((1ull << B) - 1ull));
   }
   __device__ __forceinline__ bool push(uint32_t tid, uint32_t v) const {
     gcq::gwf::NoProbe np;
-    gcq::gwf::RingOps<> o(r, tid, &np);
+    gcq::gwf::RingOps<gcq::gwf::NoProbe, true> o(r, tid, &np);
     o.enqueue(v);
     return true;
   }
   __device__ __forceinline__ bool pop(uint32_t tid, uint32_t* v) const {
     gcq::gwf::NoProbe np;
-    gcq::gwf::RingOps<> o(r, tid, &np);
+    gcq::gwf::RingOps<gcq::gwf::NoProbe, true> o(r, tid, &np);
     const uint32_t x = o.dequeue(true);
     if (x == gcq::gwf::kBot) return false;
     *v = x;

@@ -132,6 +132,15 @@ __device__ __forceinline__ uint64_t* slots(Ring* r) { return (uint64_t*)((char*)
 __device__ __forceinline__ uint64_t* tail(Ring* r) { return (uint64_t*)((char*)r + r->g.off_tail); }
 __device__ __forceinline__ uint64_t* head(Ring* r) { return (uint64_t*)((char*)r + r->g.off_head); }
 __device__ __forceinline__ int32_t* thr(Ring* r) { return (int32_t*)((char*)r + r->g.off_thr); }
+// The geometry through the constant address space when kC (ld_const): the
+// BFS frontiers' ring operations use it; the throughput, history and FIFO
+// harnesses keep plain reads (measured: constant reads cost the split 25/75
+// run 15% at 65,536 threads with combined claims).
+template <bool kC>
+__device__ __forceinline__ Geometry geom(Ring* r) {
+  if constexpr (kC) return ld_const(&r->g);
+  else return r->g;
+}
 __device__ __forceinline__ Record* rec(Ring* r, uint32_t tid) {
   return (Record*)((char*)r + r->g.off_records) + tid;
 }
@@ -211,15 +220,29 @@ __device__ inline void init_slice(Ring* r, const InitParams& p, bool full, uint3
 }
 
 // ------------------------------------------------------------------ ring ops
-template <class Probe = NoProbe>
+template <class Probe = NoProbe, bool kC = false>
 struct RingOps {
   Ring* r;
   Geometry g;
   uint32_t tid;  // caller's thread id, < g.max_threads, unique among users
   Probe* pr;
 
-  __device__ __forceinline__ RingOps(Ring* ring, uint32_t t, Probe* p) : r(ring), g(ring->g), tid(t), pr(p) {}
+  __device__ __forceinline__ RingOps(Ring* ring, uint32_t t, Probe* p) : r(ring), g(geom<kC>(ring)), tid(t), pr(p) {}
 
+  // The ring's words: at the offsets of the geometry copy g (kC), or with
+  // the header read again (the harness path).
+  __device__ __forceinline__ uint64_t* tail_(Ring* x) const {
+    if constexpr (kC) return (uint64_t*)((char*)x + g.off_tail);
+    else return tail(x);
+  }
+  __device__ __forceinline__ uint64_t* head_(Ring* x) const {
+    if constexpr (kC) return (uint64_t*)((char*)x + g.off_head);
+    else return head(x);
+  }
+  __device__ __forceinline__ int32_t* thr_(Ring* x) const {
+    if constexpr (kC) return (int32_t*)((char*)x + g.off_thr);
+    else return thr(x);
+  }
   __device__ __forceinline__ uint64_t null_tid() const { return (1ull << g.thr_bits) - 1ull; }
   __device__ __forceinline__ uint64_t cnt_of(uint64_t w) const { return w >> g.thr_bits; }
   __device__ __forceinline__ uint32_t tid_of(uint64_t w) const { return (uint32_t)(w & null_tid()); }
@@ -239,18 +262,18 @@ struct RingOps {
 
   __device__ __forceinline__ void catchup(uint64_t t, uint64_t h) {  // gwf_ring.hpp:348-371
     for (int i = 0; i < 32; ++i) {
-      const uint64_t cur = ld_rlx(tail(r));
+      const uint64_t cur = ld_rlx(tail_(r));
       const uint64_t cc = cnt_of(cur);
       if (cc >= h) break;
       if (cc != t) {
         t = cc;
-        h = cnt_of(ld_rlx(head(r)));
+        h = cnt_of(ld_rlx(head_(r)));
         if (t >= h) break;
         continue;
       }
-      if (cas_once(tail(r), cur, pack_g(h, tid_of(cur)))) break;
-      h = cnt_of(ld_rlx(head(r)));
-      t = cnt_of(ld_rlx(tail(r)));
+      if (cas_once(tail_(r), cur, pack_g(h, tid_of(cur)))) break;
+      h = cnt_of(ld_rlx(head_(r)));
+      t = cnt_of(ld_rlx(tail_(r)));
       if (t >= h) break;
     }
   }
@@ -261,7 +284,7 @@ struct RingOps {
   // with the workgroup's other enqueuing waves (group_faa).
   __device__ __forceinline__ uint64_t try_enq(uint32_t idx, bool scq_reset = false, Combine* ct = nullptr) {
     const uint64_t step = (uint64_t)1 << g.thr_bits;
-    const uint64_t T = cnt_of(ct ? group_faa(ct, tail(r), step) : wave_faa(tail(r), step));
+    const uint64_t T = cnt_of(ct ? group_faa(ct, tail_(r), step) : wave_faa(tail_(r), step));
     // Every earlier memory operation of this wave completes before the slot
     // is touched: this orders a payload store before its index is published
     // and a payload load before its index is recycled.
@@ -279,7 +302,7 @@ struct RingOps {
     bool ok = cas_rlx(&e[j], E, e_pack(tcyc - 1u, tcyc, true, true, 0, idx));
     while (!ok) {
       pr->step();
-      if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || cnt_of(ld_rlx(head(r))) <= T)))
+      if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || cnt_of(ld_rlx(head_(r))) <= T)))
         break;
       ok = cas_rlx(&e[j], E, e_pack(fresh_note(e_note(E), tcyc), tcyc, true, true, 0, idx));
     }
@@ -289,7 +312,7 @@ struct RingOps {
     // when it already holds 3n - 1 is harmless, and a reset that lands after
     // a concurrent decrement is the same as a late reset.
     const uint64_t okm = __ballot(ok);
-    if (okm && lane_id() == first_lane(okm)) reset_threshold(thr(r), g.thresh_max, scq_reset);
+    if (okm && lane_id() == first_lane(okm)) reset_threshold(thr_(r), g.thresh_max, scq_reset);
     return ok ? 0 : T;
   }
 
@@ -309,12 +332,12 @@ struct RingOps {
     uint64_t base = 0;
     uint32_t k = 0;
     if (lane_id() == leader) {
-      const uint64_t hw = ld_rlx(head(r));
+      const uint64_t hw = ld_rlx(head_(r));
       vm_wait();  // Head is read before Tail
-      const uint64_t t = cnt_of(ld_rlx(tail(r))), h = cnt_of(hw);
+      const uint64_t t = cnt_of(ld_rlx(tail_(r))), h = cnt_of(hw);
       const uint64_t avail = t > h ? t - h : 0;
       k = avail < n ? (uint32_t)avail : n;
-      if (k) base = cnt_of(faa_rlx(head(r), (uint64_t)k << g.thr_bits));
+      if (k) base = cnt_of(faa_rlx(head_(r), (uint64_t)k << g.thr_bits));
     }
     k = bcast32(k, leader);
     if (k == 0) return kEmptyTicket;
@@ -333,12 +356,12 @@ struct RingOps {
     uint32_t k = 0, off = 0;
     if (lane_id() == leader) {
       off = combine_claim(cb, n, &k, &base, [&](uint32_t lanes, uint32_t* kk, uint64_t* bb) {
-        const uint64_t hw = ld_rlx(head(r));
+        const uint64_t hw = ld_rlx(head_(r));
         vm_wait();  // Head is read before Tail
-        const uint64_t t = cnt_of(ld_rlx(tail(r))), h = cnt_of(hw);
+        const uint64_t t = cnt_of(ld_rlx(tail_(r))), h = cnt_of(hw);
         const uint64_t avail = t > h ? t - h : 0;
         *kk = avail < lanes ? (uint32_t)avail : lanes;
-        *bb = *kk ? cnt_of(faa_rlx(head(r), (uint64_t)*kk << g.thr_bits)) : 0;
+        *bb = *kk ? cnt_of(faa_rlx(head_(r), (uint64_t)*kk << g.thr_bits)) : 0;
       });
     }
     k = bcast32(k, leader);
@@ -369,7 +392,7 @@ struct RingOps {
       }
       if (H == kNoTicket) return kNoTicket;
     } else {
-      H = cnt_of(wave_faa(head(r), (uint64_t)1 << g.thr_bits));
+      H = cnt_of(wave_faa(head_(r), (uint64_t)1 << g.thr_bits));
     }
     pr->step();
     pr->after_ticket();
@@ -398,14 +421,14 @@ struct RingOps {
       }
       break;
     }
-    const uint64_t T = cnt_of(ld_rlx(tail(r)));
+    const uint64_t T = cnt_of(ld_rlx(tail_(r)));
     if (T <= H + 1) {
       catchup(T, H + 1);
-      faa_rlx(thr(r), -1);
+      faa_rlx(thr_(r), -1);
       *out = kBot;
       return 0;
     }
-    if (faa_rlx(thr(r), -1) <= 0) {
+    if (faa_rlx(thr_(r), -1) <= 0) {
       *out = kBot;
       return 0;
     }
@@ -469,7 +492,7 @@ struct RingOps {
       prepare_phase2(local, cnt_local);
       ok = cas_once(globalp, pack_g(cnt_raw, null_tid()), pack_g(cnt_raw + 1, tid));
     } while (!ok);
-    if (is_deq) faa_rlx(thr(r), -1);
+    if (is_deq) faa_rlx(thr_(r), -1);
     cas_once(local, cnt_local | kInc, cnt_local);
     cas_once(globalp, pack_g(cnt_raw + 1, tid), pack_g(cnt_raw + 1, null_tid()));
     *v = cnt_local;
@@ -487,7 +510,7 @@ struct RingOps {
       pr->step();
       const uint32_t ecyc = e_cyc(P), enote = e_note(P), eidx = e_idx(P);
       if (cyc_behind(ecyc, tcyc) && cyc_behind(enote, tcyc)) {
-        if ((!e_safe(P) && cnt_of(ld_rlx(head(r))) > T) || eidx < kBot) {
+        if ((!e_safe(P) && cnt_of(ld_rlx(head_(r))) > T) || eidx < kBot) {
           // Unusable: advance Note so no other helper produces here.
           const uint64_t nv = (P & ~(0xFFull << kNoteShift)) | ((uint64_t)tcyc << kNoteShift);
           if (!cas_rlx(&e[j], P, nv)) continue;
@@ -497,7 +520,7 @@ struct RingOps {
         if (!cas_rlx(&e[j], P, prod)) continue;
         const uint64_t tl = T << 2;
         if (cas_once(&rr->localTail, tl, tl | kFin)) cas_once(&e[j], prod, prod | kEnqBit);
-        if (ld_rlx(thr(r)) != g.thresh_max) xchg_rlx(thr(r), g.thresh_max);
+        if (ld_rlx(thr_(r)) != g.thresh_max) xchg_rlx(thr_(r), g.thresh_max);
         return true;
       }
       if (ecyc != tcyc) return false;
@@ -535,9 +558,9 @@ struct RingOps {
         val = e_pack(fresh_note(nnote, hcyc), hcyc, e_safe(P), true, 0, kBot);
       }
       if (cyc_behind(ecyc, hcyc) && !cas_rlx(&e[j], P, val)) continue;
-      const uint64_t T = cnt_of(ld_rlx(tail(r)));
+      const uint64_t T = cnt_of(ld_rlx(tail_(r)));
       if (T <= H + 1) catchup(T, H + 1);
-      if (ld_rlx(thr(r)) < 0) {
+      if (ld_rlx(thr_(r)) < 0) {
         const uint64_t hl = H << 2;
         cas_once(&rr->localHead, hl, hl | kFin);
         return true;
@@ -548,13 +571,13 @@ struct RingOps {
 
   __device__ void enqueue_slow(uint64_t v, uint32_t idx, uint32_t owner) {
     Record* rr = rec(r, owner);
-    while (slow_faa(tail(r), &rr->localTail, &v, false)) {
+    while (slow_faa(tail_(r), &rr->localTail, &v, false)) {
       if (try_enq_slow(v >> 2, idx, owner)) break;
     }
   }
   __device__ void dequeue_slow(uint64_t v, uint32_t owner) {
     Record* rr = rec(r, owner);
-    while (slow_faa(head(r), &rr->localHead, &v, true)) {
+    while (slow_faa(head_(r), &rr->localHead, &v, true)) {
       if (try_deq_slow(v >> 2, owner)) break;
     }
   }
@@ -647,7 +670,7 @@ struct RingOps {
     const uint32_t wv = ((threadIdx.x + threadIdx.y * blockDim.x) / kWave) & 15u;
     bool bounded = idle_check;
     if (!idle_check) {
-      if (ld_rlx(thr(r)) < 0) {
+      if (ld_rlx(thr_(r)) < 0) {
         s_low[wv] = low_next(s_low[wv], true);
         return kBot;
       }
