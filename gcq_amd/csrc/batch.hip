@@ -315,9 +315,14 @@ __global__ void k_fill_done(int32_t* p, uint32_t n) {
 
 // Stream-ordered launch sequence of one batch (capturable: kernels only).
 template <class F>
-int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq, const bk::Dev& d, uint64_t total_v, uint32_t max_nv,
+int enqueue_batch(gcq_batch* b, const gcqf::Frontiers<F>& fq_in, const bk::Dev& d, uint64_t total_v, uint32_t max_nv,
                   const gcq_batch_config* cfg) {
   hipStream_t s = b->stream;
+  // Both batched kernels drain their frontiers by counted pops only: G-LFQ
+  // pushes store first-lap entries.
+  gcqf::Frontiers<F> fq = fq_in;
+  if constexpr (std::is_same_v<F, gcqf::FLfq>)
+    for (auto& x : fq.q) x.counted_push = 1u;
   const uint32_t nq = d.nslots;
   hipLaunchKernelGGL(gcqf::kf_fill_i32, dim3(gcqf::grid_of(total_v)), dim3(256), 0, s, b->depth, total_v, -1);
   hipLaunchKernelGGL(gcqf::kf_zero_u64, dim3(gcqf::grid_of(b->visited_used / 2)), dim3(256), 0, s,

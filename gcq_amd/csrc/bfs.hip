@@ -589,6 +589,9 @@ int traverse(gcq_graph* g, const bfsk::Frontiers<F>& fq, const gcq_bfs_config* c
       if constexpr (std::is_same_v<F, gcqf::FSfq>)
         if (c.fs.capacity >= (uint64_t)g->host.nv)
           for (auto& x : fqp.q) x.single_lap = 1u;
+      // Every pop of these modes is counted: G-LFQ pushes store first-lap entries.
+      if constexpr (std::is_same_v<F, gcqf::FLfq>)
+        for (auto& x : fqp.q) x.counted_push = 1u;
       GCQ_HIP(hipEventRecord(e0, g->stream));
       if (bitmap)
         hipLaunchKernelGGL((bfsk::k_persistent<F, true>), dim3(grid), dim3(cfg->block), 0, g->stream, fqp, g->d_off,

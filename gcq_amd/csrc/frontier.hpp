@@ -105,6 +105,10 @@ struct FLfq {
   static constexpr bool kCounted = true;
   static constexpr bool kHeadFromDrain = true;
   gcq::lfq::Ring* r;
+  // Set by level-synchronous callers whose pops are all counted (they take
+  // only tickets pushed before the level's barrier): a push then stores a
+  // first-lap ticket's entry instead of CASing it.
+  uint32_t counted_push = 0;
   __device__ __forceinline__ void recycle(uint64_t, uint64_t, uint32_t) const {}
   __device__ __forceinline__ void rewind() const {}
   // End of the level's tickets: the input ring's Tail, which no one moves
@@ -142,7 +146,8 @@ struct FLfq {
   }
   __device__ __forceinline__ void drained(uint64_t end) const { gcq::st_rlx(gcq::lfq::head_p<true>(r), end); }
   __device__ __forceinline__ bool push(uint32_t, uint32_t v) const {
-    gcq::lfq::enqueue<gcq::lfq::NoProbe, true>(r, v);
+    if (counted_push) gcq::lfq::enqueue<gcq::lfq::NoProbe, true, true>(r, v);
+    else gcq::lfq::enqueue<gcq::lfq::NoProbe, true>(r, v);
     return true;
   }
   __device__ __forceinline__ bool pop(uint32_t, uint32_t* v) const {
@@ -683,7 +688,7 @@ struct FrontierSet {
     Frontiers<F> f;
     for (int i = 0; i < 3; ++i) {
       if constexpr (std::is_same_v<F, FGwf>) f.q[i] = FGwf{(gcq::gwf::Ring*)q[i % 2]};
-      else if constexpr (std::is_same_v<F, FLfq>) f.q[i] = FLfq{(gcq::lfq::Ring*)q[i % 2]};
+      else if constexpr (std::is_same_v<F, FLfq>) f.q[i] = FLfq{(gcq::lfq::Ring*)q[i % 2], 0u};
       else if constexpr (std::is_same_v<F, FSfq>) f.q[i] = FSfq{(gcq::sfq::Queue*)q[i % 2], 0u};
       else f.q[i] = FYmc{(gcq::ymc::Queue*)q[i], (gcq::ymc::Handle*)h[i]};
     }

@@ -210,7 +210,11 @@ struct NoProbe {
 // scq_reset: reset the threshold in SCQ's load-then-store form (the
 // free-index ring; reset_threshold).  ct: draw the Tail ticket together with
 // the workgroup's other enqueuing waves (group_faa).
-template <class Probe = NoProbe, bool kC = false>
+// kFirstLapStore: the ring's consumers only take tickets whose enqueues
+// completed before (level-synchronous frontiers drained by counted pops),
+// so no dequeuer marks a slot ahead of its enqueuer; a first-lap ticket's
+// slot then still holds its init entry, and a store replaces the CAS.
+template <class Probe = NoProbe, bool kC = false, bool kFirstLapStore = false>
 __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool scq_reset = false,
                                Combine* ct = nullptr) {
   const Geometry g = geom<kC>(r);
@@ -229,7 +233,13 @@ __device__ inline void enqueue(Ring* r, uint32_t idx, Probe* pr = nullptr, bool 
     // on the first lap.  When it does, one CAS replaces load + CAS;
     // otherwise the CAS returns the current word and the usual checks run.
     uint64_t E = (T >> g.slots_log2) == 1 ? kInit : e_pack(tcyc - 1u, true, kBotC);
-    bool ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));  // E refreshed on failure
+    bool ok;
+    if (kFirstLapStore && (T >> g.slots_log2) == 1) {
+      st_rlx(&e[j], e_pack(tcyc, true, idx));
+      ok = true;
+    } else {
+      ok = cas_rlx(&e[j], E, e_pack(tcyc, true, idx));  // E refreshed on failure
+    }
     while (!ok) {
       if (pr) pr->step();
       if (!(cyc_behind(e_cyc(E), tcyc) && e_idx(E) >= kBot && (e_safe(E) || ld_rlx(head_p<kC>(r)) <= T))) break;
